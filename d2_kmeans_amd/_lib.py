@@ -1,0 +1,93 @@
+"""ctypes binding of libd2mi.so (include/d2mi.h).
+
+The library is built in-tree by ``d2_kmeans_amd.build``; loading fails loudly
+when it is missing -- there is no CPU fallback."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libd2mi.so")
+
+D2MI_ENGINE = 0
+D2MI_PLAIN = 1
+
+_ERRNAMES = {-1: "EINVAL", -2: "EHIP", -3: "ENOMEM", -4: "EEMPTY", -5: "ENUMERIC", -6: "EUNSUPPORTED"}
+
+
+class D2MIError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"d2mi error {code} ({_ERRNAMES.get(code, '?')}): {msg}")
+        self.code = code
+
+
+class BadmmStats(C.Structure):
+    _fields_ = [("rho", C.c_double), ("obj", C.c_double), ("n_trace", C.c_int),
+                ("trace", (C.c_double * 4) * 64), ("ms_total", C.c_double),
+                ("ms_plan", C.c_double), ("n_plan_launches", C.c_int),
+                ("plan_elems", C.c_double)]
+
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_sp = C.POINTER(C.c_size_t)
+_cp = C.c_char_p
+_vp = C.c_void_p
+
+# every function the header declares, with its argument types
+SIGNATURES = {
+    "d2mi_last_error": (C.c_char_p, []),
+    "d2mi_device_count": (C.c_int, [_ip]),
+    "d2mi_create": (C.c_int, [C.POINTER(_vp), C.c_int]),
+    "d2mi_destroy": (C.c_int, [_vp]),
+    "d2mi_synchronize": (C.c_int, [_vp]),
+    "d2mi_set_phase": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_size_t, C.c_size_t, _ip, _sp,
+                                 _dp, _dp, _ip, C.c_int, _dp, _dp]),
+    "d2mi_set_labels": (C.c_int, [_vp, _ip, _vp]),
+    "d2mi_get_labels": (C.c_int, [_vp, _ip, _vp]),
+    "d2mi_set_centroids": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _dp, _dp]),
+    "d2mi_get_centroids": (C.c_int, [_vp, C.c_int, _dp, _dp]),
+    "d2mi_badmm": (C.c_int, [_vp, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int,
+                             C.POINTER(BadmmStats)]),
+    "d2mi_get_plan": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _dp]),
+    "d2mi_set_plan": (C.c_int, [_vp, C.c_int, C.c_int, _dp]),
+    "d2mi_pdist2": (C.c_int, [_vp, C.c_int, C.c_size_t, C.c_size_t, _dp, _dp, _dp]),
+}
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built: run `python -m d2_kmeans_amd.build` "
+                               "(there is no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        raise D2MIError(rc, lib().d2mi_last_error().decode(errors="replace"))
+    return rc
+
+
+def ptr(a, t=_dp):
+    if a is None:
+        return None
+    return a.ctypes.data_as(t)
+
+
+def vptr(a):
+    if a is None:
+        return None
+    return C.c_void_p(a.ctypes.data)

@@ -1,0 +1,895 @@
+// B-ADMM centroid update on the device (the reference's d2_centroid_sphBregman,
+// centroid_Bregman.c:51-325).
+//
+// Data flow of one update (all on one stream, no host round trip between
+// inner iterations):
+//   1. stable counting sort of the objects by label -> perm, segment starts
+//      (label counts double as the empty-cluster check of :138-144);
+//   2. setup: sum of the ground costs (rho, :109), Z re-initialised for
+//      switched objects (:116-124), Y = 0 (:128);
+//   3. per inner iteration (:150): the plan kernel does steps 1-3 for every
+//      object (X, Z, Y updates, :155-175) in one HBM pass over Z and Y, with the
+//      ground cost computed on the fly from the supports, and emits per-label
+//      FP64 partial sums of the normalised row sums (step 4, :179-185) and, on
+//      support-update iterations, of X * supports and X row sums (step 5,
+//      :196-212); the finalize kernel sums the partials in a fixed order and
+//      normalises (:190, :219-221).
+// Objects are processed in label-sorted order, so a workgroup accumulates
+// per-label sums in LDS and writes one record per (workgroup, label) pair;
+// record (g, l) lives at index g + l, which is unique because chunk label
+// ranges do not interleave.  Every sum has a fixed order: results are
+// bit-reproducible run to run.
+#include <cmath>
+#include <type_traits>
+
+#include "d2mi_internal.h"
+
+namespace {
+
+constexpr int NT = 256;        // threads per workgroup (4 waves of 64)
+constexpr int MAXD_FUSED = 8;  // Euclidean cost fused on the fly for d <= 8
+constexpr int FLAG_BADLABEL = 1, FLAG_EMPTY = 2, FLAG_NUMERIC = 4;
+
+enum CostMode { COST_EUCLID = 0, COST_HIST = 1, COST_SPARSE = 2 };
+
+template <typename T>
+__device__ __forceinline__ T dexp(T x);
+template <>
+__device__ __forceinline__ float dexp<float>(float x) { return expf(x); }
+template <>
+__device__ __forceinline__ double dexp<double>(double x) { return exp(x); }
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Block-wide sum with a fixed association order; every thread gets the
+// result.  Must be called by all threads of the block.
+__device__ double block_sum(double v, double *red) {
+  v = wave_sum(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0.0;
+  for (int i = 0; i < NT / 64; ++i) t += red[i];
+  return t;
+}
+
+__host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~size_t(15); }
+
+// LDS layout of the plan kernel, shared by host (sizing) and device.
+struct PlanLds {
+  size_t A, B, X, f, w, xs, cs, cw, g, rr, ids, acc, red, total;
+  __host__ __device__ PlanLds(int tsize, int sp, int m_max, int s, int d, int keepx, int R) {
+    const size_t EP = (size_t)sp * m_max;
+    size_t o = 0;
+    A = o; o = align16(o + EP * tsize);
+    B = o; o = align16(o + EP * tsize);
+    X = o; o = align16(o + (keepx ? EP : 0) * tsize);
+    f = o; o = align16(o + (size_t)m_max * tsize);
+    w = o; o = align16(o + (size_t)m_max * tsize);
+    xs = o; o = align16(o + (size_t)m_max * (d > 0 ? d : 1) * tsize);
+    cs = o; o = align16(o + (size_t)s * (d > 0 ? d : 1) * tsize);
+    cw = o; o = align16(o + (size_t)s * tsize);
+    g = o; o = align16(o + (size_t)s * tsize);
+    rr = o; o = align16(o + (size_t)s * tsize);
+    ids = o; o = align16(o + (size_t)m_max * sizeof(int));
+    acc = o; o = align16(o + (size_t)R * sizeof(double));
+    red = o; o = align16(o + 8 * sizeof(double));
+    total = o;
+  }
+};
+
+template <typename T>
+struct PlanArgs {
+  int N, s, d, K, sp, m_max, chunk, R, V;
+  int upd, resid, keepx;
+  const int *__restrict__ p_str;
+  const long long *__restrict__ cum;
+  const int *__restrict__ perm;
+  const int *__restrict__ label;
+  const T *__restrict__ w;
+  const T *__restrict__ supp;
+  const int *__restrict__ sym;
+  const T *__restrict__ dist;
+  const T *__restrict__ cw;
+  const T *__restrict__ csupp;
+  const double *__restrict__ scal;  // [0] = rho, [1] = 1/rho
+  T *__restrict__ Z;
+  T *__restrict__ Y;
+  double *__restrict__ rec;         // [(G + K) * R]
+  double *__restrict__ resid_part;  // [G * 3]
+};
+
+// ground cost of element (k, j) before the division by rho
+template <typename T, int CM>
+__device__ __forceinline__ T cost_raw(int k, int j, int s, int d, int V, const T *cs,
+                                      const T *xs, const int *ids, const T *__restrict__ dist) {
+  if constexpr (CM == COST_EUCLID) {
+    T acc = 0;
+    for (int dd = 0; dd < d; ++dd) {
+      const T u = cs[k * d + dd] - xs[j * d + dd];
+      acc += u * u;
+    }
+    return acc;
+  } else if constexpr (CM == COST_HIST) {
+    return dist[k + (size_t)j * s];
+  } else {
+    return dist[(size_t)ids[j] * V + k];
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ T scale_cost(T c, double rho, T inv_rho) {
+  if constexpr (std::is_same<T, double>::value) return c / rho;  // C[i] /= rho (:110)
+  else return c * inv_rho;
+}
+
+template <typename T, int CM>
+__global__ __launch_bounds__(NT) void k_badmm_plan(PlanArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr bool F64 = std::is_same<T, double>::value;
+  const int s = a.s, d = a.d, sp = a.sp, tid = threadIdx.x;
+  const PlanLds L(sizeof(T), sp, a.m_max, s, d, a.keepx, a.R);
+  T *A = (T *)(smem + L.A), *B = (T *)(smem + L.B), *Xs = (T *)(smem + L.X);
+  T *f = (T *)(smem + L.f), *wl = (T *)(smem + L.w), *xs = (T *)(smem + L.xs);
+  T *cs = (T *)(smem + L.cs), *cwl = (T *)(smem + L.cw), *g = (T *)(smem + L.g);
+  T *rr = (T *)(smem + L.rr);
+  int *ids = (int *)(smem + L.ids);
+  double *acc = (double *)(smem + L.acc), *red = (double *)(smem + L.red);
+
+  const int gi = blockIdx.x;
+  const int p0 = gi * a.chunk, p1 = min(a.N, p0 + a.chunk);
+  const double rho = a.scal[0];
+  const T inv_rho = (T)a.scal[1];
+  const int dk = NT % s, dj = NT / s, k0 = tid % s, j0 = tid / s;
+  double r_obj = 0.0, r_pr = 0.0, r_dr = 0.0;
+  for (int q = tid; q < a.R; q += NT) acc[q] = 0.0;
+  int cur = -1;
+
+  for (int p = p0; p < p1; ++p) {
+    const int o = a.perm[p];
+    const int l = a.label[o];
+    const int m = a.p_str[o];
+    const long long off = a.cum[o];
+    if (l != cur) {
+      __syncthreads();  // acc of the previous label is complete
+      if (cur >= 0)
+        for (int q = tid; q < a.R; q += NT) {
+          a.rec[(size_t)(gi + cur) * a.R + q] = acc[q];
+          acc[q] = 0.0;
+        }
+      cur = l;
+      for (int q = tid; q < s; q += NT) cwl[q] = a.cw[(size_t)l * s + q];
+      if constexpr (CM == COST_EUCLID)
+        for (int q = tid; q < s * d; q += NT) cs[q] = a.csupp[(size_t)l * s * d + q];
+    }
+    for (int q = tid; q < m; q += NT) wl[q] = a.w[off + q];
+    if constexpr (CM == COST_EUCLID)
+      for (int q = tid; q < m * d; q += NT) xs[q] = a.supp[(size_t)off * d + q];
+    if constexpr (CM == COST_SPARSE)
+      for (int q = tid; q < m; q += NT) ids[q] = a.sym[off + q];
+    __syncthreads();
+
+    const int E = s * m;
+    const size_t base = (size_t)s * off;
+    // step 1a: t = Z exp(-(C + Y)) + 1e-9 (:155-157)
+    {
+      int k = k0, j = j0;
+      for (int e = tid; e < E; e += NT) {
+        const T c = scale_cost<T>(cost_raw<T, CM>(k, j, s, d, a.V, cs, xs, ids, a.dist), rho,
+                                  inv_rho);
+        const T z = a.Z[base + e], y = a.Y[base + e];
+        const int li = k + j * sp;
+        A[li] = z * dexp<T>(-(c + y)) + (T)1e-9;
+        B[li] = y;
+        k += dk;
+        j += dj;
+        if (k >= s) { k -= s; ++j; }
+      }
+    }
+    __syncthreads();
+    // step 1b: column sums (_dcnorm, :158)
+    for (int j = tid; j < m; j += NT) {
+      T acc_c = 0;
+      const T *colp = A + j * sp;
+      for (int k = 0; k < s; ++k) acc_c += colp[k];
+      f[j] = F64 ? acc_c : wl[j] / acc_c;
+    }
+    __syncthreads();
+    // step 1c + 2a: x = t / colsum * w (:158-159); z' = x exp(Y) + 1e-9 (:166)
+    {
+      int k = k0, j = j0;
+      for (int e = tid; e < E; e += NT) {
+        const int li = k + j * sp;
+        const T x = F64 ? (A[li] / f[j]) * wl[j] : A[li] * f[j];
+        const T y = B[li];
+        A[li] = x * dexp<T>(y) + (T)1e-9;
+        if (a.keepx) Xs[li] = x;
+        else B[li] = y + x;
+        k += dk;
+        j += dj;
+        if (k >= s) { k -= s; ++j; }
+      }
+    }
+    __syncthreads();
+    // step 2b: row sums (_drnorm, :169); step 5 partials on update iterations
+    for (int k = tid; k < s; k += NT) {
+      T r = 0;
+      for (int j = 0; j < m; ++j) r += A[k + j * sp];
+      rr[k] = r;
+      g[k] = F64 ? r : cwl[k] / r;
+    }
+    if (a.upd) {
+      for (int k = tid; k < s; k += NT) {
+        double P[MAXD_FUSED];
+        for (int dd = 0; dd < d; ++dd) P[dd] = 0.0;
+        double rx = 0.0;
+        for (int j = 0; j < m; ++j) {
+          const double x = (double)Xs[k + j * sp];
+          rx += x;  // row sums of X (:211)
+          for (int dd = 0; dd < d; ++dd) P[dd] += x * (double)xs[j * d + dd];  // supp X^T (:206)
+        }
+        for (int dd = 0; dd < d; ++dd) acc[s + k * d + dd] += P[dd];
+        acc[s + s * d + k] += rx;
+      }
+    }
+    __syncthreads();
+    // step 4: normalised row sums summed per label (:180-185)
+    double rsum = 0.0;
+    for (int k = tid; k < s; k += NT) rsum += (double)rr[k];
+    rsum = block_sum(rsum, red);
+    for (int k = tid; k < s; k += NT) acc[k] += (double)rr[k] / rsum;
+    // step 2c + 3: z = z' / rowsum * c_w (:169-170); Y += X - Z (:175)
+    {
+      int k = k0, j = j0;
+      for (int e = tid; e < E; e += NT) {
+        const int li = k + j * sp;
+        const T z = F64 ? (A[li] / g[k]) * cwl[k] : A[li] * g[k];
+        if (a.resid) {  // step 6 (:300-305), and the exit objective
+          const T c = scale_cost<T>(cost_raw<T, CM>(k, j, s, d, a.V, cs, xs, ids, a.dist), rho,
+                                    inv_rho);
+          const T x = Xs[li];
+          const T z0 = a.Z[base + e];
+          r_obj += (double)c * (double)x;
+          r_pr += fabs((double)x - (double)z);
+          r_dr += fabs((double)z0 - (double)z);
+        }
+        a.Z[base + e] = z;
+        a.Y[base + e] = a.keepx ? B[li] + (Xs[li] - z) : B[li] - z;
+        k += dk;
+        j += dj;
+        if (k >= s) { k -= s; ++j; }
+      }
+    }
+    __syncthreads();
+  }
+  __syncthreads();
+  if (cur >= 0)
+    for (int q = tid; q < a.R; q += NT) a.rec[(size_t)(gi + cur) * a.R + q] = acc[q];
+  if (a.resid) {
+    const double t0 = block_sum(r_obj, red);
+    const double t1 = block_sum(r_pr, red);
+    const double t2 = block_sum(r_dr, red);
+    if (tid == 0) {
+      a.resid_part[gi * 3 + 0] = t0;
+      a.resid_part[gi * 3 + 1] = t1;
+      a.resid_part[gi * 3 + 2] = t2;
+    }
+  }
+}
+
+// ------------------------------------------------------------- finalize
+
+template <typename T>
+struct FinArgs {
+  int K, s, d, chunk, R, G, upd, resid, iter;
+  const int *__restrict__ seg;
+  const double *__restrict__ rec;
+  const double *__restrict__ resid_part;
+  double *__restrict__ cw64;
+  T *__restrict__ cwT;
+  double *__restrict__ cs64;
+  T *__restrict__ csT;
+  double *__restrict__ resid_out;  // [iters][3]
+  int *__restrict__ flags;
+};
+
+template <typename T>
+__global__ __launch_bounds__(NT) void k_badmm_finalize(FinArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  double *sums = (double *)smem;
+  double *red = sums + a.R;
+  const int l = blockIdx.x, tid = threadIdx.x, s = a.s, d = a.d;
+  const int lo = a.seg[l], hi = a.seg[l + 1];
+  const int glo = lo / a.chunk, ghi = (hi - 1) / a.chunk;
+  for (int q = tid; q < a.R; q += NT) {
+    double v = 0.0;
+    for (int g = glo; g <= ghi; ++g) v += a.rec[(size_t)(g + l) * a.R + q];
+    sums[q] = v;
+  }
+  __syncthreads();
+  double tot = 0.0;
+  for (int k = tid; k < s; k += NT) tot += sums[k];
+  tot = block_sum(tot, red);
+  if (!(tot > 0.0)) {
+    if (tid == 0) atomicOr(a.flags, FLAG_NUMERIC);
+  } else {
+    for (int k = tid; k < s; k += NT) {
+      const double v = sums[k] / tot;  // _dcnorm over the centroid (:190)
+      a.cw64[(size_t)l * s + k] = v;
+      a.cwT[(size_t)l * s + k] = (T)v;
+    }
+  }
+  if (a.upd) {
+    for (int q = tid; q < s * d; q += NT) {
+      const int k = q / d;
+      const double rx = sums[s + s * d + k];
+      if (!(rx > 0.0)) {
+        atomicOr(a.flags, FLAG_NUMERIC);
+        continue;
+      }
+      const double v = sums[s + q] / rx;  // _dirms (:219-221)
+      a.cs64[(size_t)l * s * d + q] = v;
+      a.csT[(size_t)l * s * d + q] = (T)v;
+    }
+  }
+  if (a.resid && l == 0) {
+    for (int c = 0; c < 3; ++c) {
+      double v = 0.0;
+      for (int g = tid; g < a.G; g += NT) v += a.resid_part[g * 3 + c];
+      v = block_sum(v, red);
+      if (tid == 0) a.resid_out[a.iter * 3 + c] = v;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- setup
+
+template <typename T>
+struct SetupArgs {
+  int N, s, d, V, m_max, per_block;
+  const int *__restrict__ p_str;
+  const long long *__restrict__ cum;
+  const int *__restrict__ label;
+  const char *__restrict__ lsw;
+  int force_init;
+  const T *__restrict__ w;
+  const T *__restrict__ supp;
+  const int *__restrict__ sym;
+  const T *__restrict__ dist;
+  const T *__restrict__ cw;
+  const T *__restrict__ csupp;
+  T *__restrict__ Z;
+  T *__restrict__ Y;
+  double *__restrict__ part;  // [gridDim.x]
+};
+
+// Ground-cost sum (rho, :109), Z re-initialised as w_i[j] * c_w[label][k]
+// for switched objects (:116-124), Y = 0 (:128).
+template <typename T, int CM>
+__global__ __launch_bounds__(NT) void k_badmm_setup(SetupArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int s = a.s, d = a.d, tid = threadIdx.x;
+  T *xs = (T *)smem;
+  T *cs = xs + (size_t)a.m_max * (d > 0 ? d : 1);
+  int *ids = (int *)(cs + (size_t)s * (d > 0 ? d : 1));
+  double *red = (double *)(smem + align16((size_t)((char *)(ids + a.m_max) - (char *)smem)));
+  double csum = 0.0;
+  const int o0 = blockIdx.x * a.per_block, o1 = min(a.N, o0 + a.per_block);
+  for (int o = o0; o < o1; ++o) {
+    const int l = a.label[o], m = a.p_str[o];
+    const long long off = a.cum[o];
+    __syncthreads();
+    if constexpr (CM == COST_EUCLID) {
+      for (int q = tid; q < m * d; q += NT) xs[q] = a.supp[(size_t)off * d + q];
+      for (int q = tid; q < s * d; q += NT) cs[q] = a.csupp[(size_t)l * s * d + q];
+    }
+    if constexpr (CM == COST_SPARSE)
+      for (int q = tid; q < m; q += NT) ids[q] = a.sym[off + q];
+    __syncthreads();
+    const size_t base = (size_t)s * off;
+    const bool init = a.force_init || a.lsw[o] == 1;
+    for (int e = tid; e < s * m; e += NT) {
+      const int j = e / s, k = e - j * s;
+      const T c = cost_raw<T, CM>(k, j, s, d, a.V, cs, xs, ids, a.dist);
+      csum += fabs((double)c);
+      if (init) a.Z[base + e] = a.w[off + j] * a.cw[(size_t)l * s + k];
+      a.Y[base + e] = (T)0;
+    }
+  }
+  csum = block_sum(csum, red);
+  if (tid == 0) a.part[blockIdx.x] = csum;
+}
+
+__global__ void k_badmm_rho(const double *__restrict__ part, int nparts, double rhoCoeff,
+                            double denom, double *__restrict__ scal, int *flags) {
+  __shared__ double red[8];
+  double v = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += NT) v += part[i];
+  v = block_sum(v, red);
+  if (threadIdx.x == 0) {
+    const double rho = rhoCoeff * v / denom;
+    scal[0] = rho;
+    scal[1] = 1.0 / rho;
+    if (!(rho > 0.0)) atomicOr(flags, FLAG_NUMERIC);
+  }
+}
+
+// --------------------------------------------- stable counting sort by label
+
+__global__ __launch_bounds__(NT) void k_label_hist(const int *__restrict__ label, int N, int K,
+                                                   int *__restrict__ blkcnt, int *flags) {
+  extern __shared__ int hist[];
+  for (int l = threadIdx.x; l < K; l += NT) hist[l] = 0;
+  __syncthreads();
+  const int o = blockIdx.x * NT + threadIdx.x;
+  if (o < N) {
+    const int l = label[o];
+    if (l < 0 || l >= K) atomicOr(flags, FLAG_BADLABEL);
+    else atomicAdd(&hist[l], 1);
+  }
+  __syncthreads();
+  for (int l = threadIdx.x; l < K; l += NT) blkcnt[(size_t)blockIdx.x * K + l] = hist[l];
+}
+
+// one block per label: exclusive scan over blocks; totals into seg[l]
+__global__ __launch_bounds__(NT) void k_label_scan(int nb, int K, int *__restrict__ blkcnt,
+                                                   int *__restrict__ total) {
+  __shared__ int buf[NT];
+  __shared__ int carry;
+  const int l = blockIdx.x;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int b0 = 0; b0 < nb; b0 += NT) {
+    const int b = b0 + threadIdx.x;
+    const int v = b < nb ? blkcnt[(size_t)b * K + l] : 0;
+    buf[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < NT; off <<= 1) {  // inclusive Hillis-Steele scan
+      const int t = threadIdx.x >= off ? buf[threadIdx.x - off] : 0;
+      __syncthreads();
+      buf[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (b < nb) blkcnt[(size_t)b * K + l] = carry + buf[threadIdx.x] - v;
+    __syncthreads();
+    if (threadIdx.x == NT - 1) carry += buf[NT - 1];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) total[l] = carry;
+}
+
+__global__ void k_label_seg(int K, const int *__restrict__ total, int *__restrict__ seg,
+                            int *flags) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    int run = 0;
+    for (int l = 0; l < K; ++l) {
+      seg[l] = run;
+      if (total[l] == 0) atomicOr(flags, FLAG_EMPTY);
+      run += total[l];
+    }
+    seg[K] = run;
+  }
+}
+
+__global__ __launch_bounds__(NT) void k_label_scatter(const int *__restrict__ label, int N, int K,
+                                                      const int *__restrict__ blkoff,
+                                                      const int *__restrict__ seg,
+                                                      int *__restrict__ perm) {
+  __shared__ int lab[NT];
+  const int o = blockIdx.x * NT + threadIdx.x;
+  lab[threadIdx.x] = o < N ? label[o] : -1;
+  __syncthreads();
+  if (o < N) {
+    const int l = lab[threadIdx.x];
+    if (l < 0 || l >= K) return;
+    int rank = 0;
+    for (int t = 0; t < (int)threadIdx.x; ++t) rank += lab[t] == l;
+    perm[seg[l] + blkoff[(size_t)blockIdx.x * K + l] + rank] = o;
+  }
+}
+
+// ------------------------------------------ plain one-thread-per-object form
+
+template <typename T>
+struct PlainArgs {
+  int N, s, d, V, upd, keepx;
+  const int *__restrict__ p_str;
+  const long long *__restrict__ cum;
+  const int *__restrict__ label;
+  const T *__restrict__ w;
+  const T *__restrict__ supp;
+  const int *__restrict__ sym;
+  const T *__restrict__ dist;
+  const T *__restrict__ cw;
+  const T *__restrict__ csupp;
+  const double *__restrict__ scal;
+  T *__restrict__ Z;
+  T *__restrict__ Y;
+  T *__restrict__ X;       // [s*col] scratch
+  double *__restrict__ acc;  // [K * s * (d + 2)]: weights, supports, X row sums
+};
+
+// The CPU loops of centroid_Bregman.c:155-212 with one thread owning one
+// object, state in global memory and per-label sums by FP64 atomics.
+template <typename T, int CM>
+__global__ void k_badmm_plain(PlainArgs<T> a) {
+  const int o = blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= a.N) return;
+  const int s = a.s, d = a.d, l = a.label[o], m = a.p_str[o];
+  const long long off = a.cum[o];
+  const size_t base = (size_t)s * off;
+  const double rho = a.scal[0];
+  const T inv_rho = (T)a.scal[1];
+  const T *cs = a.csupp + (size_t)l * s * d;
+  const T *xs = a.supp + (size_t)off * d;
+  const int *ids = a.sym + off;
+  for (int j = 0; j < m; ++j) {
+    T csum = 0;
+    for (int k = 0; k < s; ++k) {
+      const size_t e = base + k + (size_t)j * s;
+      const T c = scale_cost<T>(cost_raw<T, CM>(k, j, s, d, a.V, cs, xs, ids, a.dist), rho,
+                                inv_rho);
+      const T t = a.Z[e] * dexp<T>(-(c + a.Y[e])) + (T)1e-9;
+      a.X[e] = t;
+      csum += t;
+    }
+    const T wj = a.w[off + j];
+    for (int k = 0; k < s; ++k) {
+      const size_t e = base + k + (size_t)j * s;
+      const T x = (a.X[e] / csum) * wj;
+      a.X[e] = x;
+      a.Z[e] = x * dexp<T>(a.Y[e]) + (T)1e-9;
+    }
+  }
+  const size_t stride = (size_t)s * (d + 2);
+  double *acc = a.acc + (size_t)l * stride;
+  double rsum = 0.0;
+  for (int k = 0; k < s; ++k) {
+    T r = 0;
+    for (int j = 0; j < m; ++j) r += a.Z[base + k + (size_t)j * s];
+    rsum += (double)r;
+  }
+  for (int k = 0; k < s; ++k) {
+    T r = 0;
+    for (int j = 0; j < m; ++j) r += a.Z[base + k + (size_t)j * s];
+    const T ck = a.cw[(size_t)l * s + k];
+    for (int j = 0; j < m; ++j) {
+      const size_t e = base + k + (size_t)j * s;
+      const T z = (a.Z[e] / r) * ck;
+      a.Z[e] = z;
+      a.Y[e] += a.X[e] - z;
+    }
+    atomicAdd(&acc[k], (double)r / rsum);
+    if (a.upd) {
+      double rx = 0.0;
+      for (int j = 0; j < m; ++j) {
+        const double x = (double)a.X[base + k + (size_t)j * s];
+        rx += x;
+        for (int dd = 0; dd < d; ++dd)
+          atomicAdd(&acc[s + k * d + dd], x * (double)xs[j * d + dd]);
+      }
+      atomicAdd(&acc[s + s * d + k], rx);
+    }
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void k_badmm_plain_finalize(int K, int s, int d, int upd,
+                                                             double *__restrict__ acc,
+                                                             double *cw64, T *cwT, double *cs64,
+                                                             T *csT, int *flags) {
+  __shared__ double red[8];
+  const int l = blockIdx.x, tid = threadIdx.x;
+  double *A = acc + (size_t)l * s * (d + 2);
+  double tot = 0.0;
+  for (int k = tid; k < s; k += NT) tot += A[k];
+  tot = block_sum(tot, red);
+  if (!(tot > 0.0) && tid == 0) atomicOr(flags, FLAG_NUMERIC);
+  for (int k = tid; k < s; k += NT) {
+    const double v = A[k] / tot;
+    cw64[(size_t)l * s + k] = v;
+    cwT[(size_t)l * s + k] = (T)v;
+  }
+  if (upd)
+    for (int q = tid; q < s * d; q += NT) {
+      const double rx = A[s + s * d + q / d];
+      const double v = A[s + q] / rx;
+      cs64[(size_t)l * s * d + q] = v;
+      csT[(size_t)l * s * d + q] = (T)v;
+    }
+  __syncthreads();
+  for (int q = tid; q < s * (d + 2); q += NT) A[q] = 0.0;
+}
+
+__global__ void k_zero_f64(double *p, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = 0.0;
+}
+
+__global__ void k_zero_i32(int *p, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = 0;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ host side
+
+int d2mi_ensure_plans(d2mi_ctx *ctx, PhaseDev *p, int precision) {
+  const size_t n = (size_t)p->s * p->col;
+  if (precision == 64) {
+    if (p->plan_s64 != p->s) {
+      D2MI_TRY(p->Z64.ensure(n * sizeof(double)));
+      D2MI_TRY(p->Y64.ensure(n * sizeof(double)));
+      p->plan_s64 = p->s;
+      return 1;  // freshly allocated
+    }
+  } else {
+    if (p->plan_s32 != p->s) {
+      D2MI_TRY(p->Z32.ensure(n * sizeof(float)));
+      D2MI_TRY(p->Y32.ensure(n * sizeof(float)));
+      p->plan_s32 = p->s;
+      return 1;
+    }
+  }
+  return 0;
+}
+
+namespace {
+
+struct Launch {
+  int G, chunk;
+};
+
+template <typename T, int CM>
+int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int updatePerLoops,
+              int variant, d2mi_badmm_stats *st, bool fresh) {
+  constexpr bool F64 = std::is_same<T, double>::value;
+  hipStream_t S = ctx->stream;
+  const int N = (int)p->N, K = p->K, s = p->s, d = CM == COST_EUCLID ? p->dim : 0;
+  const int m_max = p->m_max;
+  T *Z = F64 ? (T *)p->Z64.p : (T *)p->Z32.p;
+  T *Y = F64 ? (T *)p->Y64.p : (T *)p->Y32.p;
+  const T *w = F64 ? (const T *)p->w64.p : (const T *)p->w32.p;
+  const T *supp = F64 ? (const T *)p->supp64.p : (const T *)p->supp32.p;
+  const T *dist = F64 ? (const T *)p->dist64.p : (const T *)p->dist32.p;
+  T *cwT = F64 ? (T *)p->cw64.p : (T *)p->cw32.p;
+  T *csT = F64 ? (T *)p->cs64.p : (T *)p->cs32.p;
+  const int *sym = p->sym.as<int>();
+  const bool euclid_upd = CM == COST_EUCLID;
+
+  // ---- scratch
+  const int nb = (N + NT - 1) / NT;
+  D2MI_TRY(ctx->perm.ensure(sizeof(int) * (size_t)N));
+  D2MI_TRY(ctx->blkcnt.ensure(sizeof(int) * (size_t)nb * K));
+  D2MI_TRY(ctx->segstart.ensure(sizeof(int) * (size_t)(2 * K + 2)));
+  D2MI_TRY(ctx->scal.ensure(sizeof(double) * 8));
+  D2MI_TRY(ctx->flags.ensure(sizeof(int) * 4));
+  D2MI_TRY(ctx->resid.ensure(sizeof(double) * 3 * (size_t)(maxIters + 1)));
+  int *seg = ctx->segstart.as<int>();
+  int *total = seg + K + 1;
+  int *flags = ctx->flags.as<int>();
+  double *scal = ctx->scal.as<double>();
+  k_zero_i32<<<1, 64, 0, S>>>(flags, 4);
+
+  // ---- 1. label sort
+  k_label_hist<<<nb, NT, sizeof(int) * K, S>>>(ctx->label.as<int>(), N, K, ctx->blkcnt.as<int>(),
+                                               flags);
+  k_label_scan<<<K, NT, 0, S>>>(nb, K, ctx->blkcnt.as<int>(), total);
+  k_label_seg<<<1, 64, 0, S>>>(K, total, seg, flags);
+  k_label_scatter<<<nb, NT, 0, S>>>(ctx->label.as<int>(), N, K, ctx->blkcnt.as<int>(), seg,
+                                    ctx->perm.as<int>());
+  D2MI_HIP(hipGetLastError());
+  int hflags = 0;
+  D2MI_HIP(hipMemcpyAsync(&hflags, flags, sizeof(int), hipMemcpyDeviceToHost, S));
+  D2MI_HIP(hipStreamSynchronize(S));
+  if (hflags & FLAG_BADLABEL) {
+    d2mi_set_error("a label is outside [0, %d)", K);
+    return D2MI_EINVAL;
+  }
+  if (hflags & FLAG_EMPTY) {
+    d2mi_set_error("a cluster has no object (the reference asserts, centroid_Bregman.c:144)");
+    return D2MI_EEMPTY;
+  }
+
+  // ---- 2. setup
+  D2MI_HIP(hipEventRecord(ctx->ev[0], S));
+  const int setup_per_block = std::max(1, N / (ctx->num_cu * 8));
+  const int setup_blocks = (N + setup_per_block - 1) / setup_per_block;
+  D2MI_TRY(ctx->part.ensure(sizeof(double) * (size_t)std::max(setup_blocks, 3 * 65536)));
+  {
+    SetupArgs<T> sa{};
+    sa.N = N; sa.s = s; sa.d = d; sa.V = p->V; sa.m_max = m_max; sa.per_block = setup_per_block;
+    sa.p_str = p->p_str.as<int>(); sa.cum = p->cum.as<long long>(); sa.label = ctx->label.as<int>();
+    sa.lsw = ctx->lsw.as<char>(); sa.force_init = fresh ? 1 : 0;
+    sa.w = w; sa.supp = supp; sa.sym = sym; sa.dist = dist; sa.cw = cwT; sa.csupp = csT;
+    sa.Z = Z; sa.Y = Y; sa.part = ctx->part.as<double>();
+    const size_t lds = align16(sizeof(T) * ((size_t)m_max * std::max(d, 1) + (size_t)s * std::max(d, 1)) +
+                               sizeof(int) * m_max) + 8 * sizeof(double);
+    k_badmm_setup<T, CM><<<setup_blocks, NT, lds, S>>>(sa);
+    k_badmm_rho<<<1, NT, 0, S>>>(ctx->part.as<double>(), setup_blocks, rhoCoeff,
+                                 (double)s * (double)p->col, scal, flags);
+    D2MI_HIP(hipGetLastError());
+  }
+
+  auto is_resid = [&](int iter) {
+    return (iter % 100 == 99) || (iter < 100 && iter % 20 == 19) || iter == maxIters;
+  };
+
+  std::vector<hipEvent_t> evs;
+  int n_launch = 0;
+  double ms_plan = 0.0;
+
+  if (variant == D2MI_BADMM_ENGINE) {
+    // ---- launch geometry: contiguous chunks of the label-sorted order
+    const int Gt = std::max(1, ctx->num_cu * 8);
+    const int chunk = std::max(1, (N + Gt - 1) / Gt);
+    const int G = (N + chunk - 1) / chunk;
+    const int sp = (s % 2 == 0) ? s + 1 : s;  // odd LDS column stride: conflict-free sums
+    const int Rmax = euclid_upd ? s * (d + 2) : s;
+    D2MI_TRY(ctx->rec.ensure(sizeof(double) * (size_t)(G + K) * Rmax));
+    D2MI_TRY(ctx->part.ensure(sizeof(double) * (size_t)std::max(setup_blocks, 3 * G)));
+    const size_t lds_full = PlanLds(sizeof(T), sp, m_max, s, d, 1, Rmax).total;
+    if (lds_full > ctx->lds_max) {
+      d2mi_set_error("object plan s*m_max = %d x %d needs %zu B of LDS (limit %zu)", s, m_max,
+                     lds_full, ctx->lds_max);
+      return D2MI_EUNSUPPORTED;
+    }
+    evs.resize(2 * (maxIters + 1));
+    for (auto &e : evs) D2MI_HIP(hipEventCreate(&e));
+    for (int iter = 0; iter <= maxIters; ++iter) {
+      const int upd = euclid_upd && (iter % updatePerLoops == 0);
+      const int resid = is_resid(iter);
+      const int keepx = F64 || upd || resid;
+      const int R = upd ? s * (d + 2) : s;
+      PlanArgs<T> pa{};
+      pa.N = N; pa.s = s; pa.d = d; pa.K = K; pa.sp = sp; pa.m_max = m_max; pa.chunk = chunk;
+      pa.R = R; pa.V = p->V; pa.upd = upd; pa.resid = resid; pa.keepx = keepx;
+      pa.p_str = p->p_str.as<int>(); pa.cum = p->cum.as<long long>(); pa.perm = ctx->perm.as<int>();
+      pa.label = ctx->label.as<int>(); pa.w = w; pa.supp = supp; pa.sym = sym; pa.dist = dist;
+      pa.cw = cwT; pa.csupp = csT; pa.scal = scal; pa.Z = Z; pa.Y = Y;
+      pa.rec = ctx->rec.as<double>(); pa.resid_part = ctx->part.as<double>();
+      const size_t lds = PlanLds(sizeof(T), sp, m_max, s, d, keepx, R).total;
+      D2MI_HIP(hipEventRecord(evs[2 * iter], S));
+      k_badmm_plan<T, CM><<<G, NT, lds, S>>>(pa);
+      D2MI_HIP(hipEventRecord(evs[2 * iter + 1], S));
+      FinArgs<T> fa{};
+      fa.K = K; fa.s = s; fa.d = d; fa.chunk = chunk; fa.R = R; fa.G = G; fa.upd = upd;
+      fa.resid = resid; fa.iter = iter; fa.seg = seg; fa.rec = ctx->rec.as<double>();
+      fa.resid_part = ctx->part.as<double>(); fa.cw64 = p->cw64.as<double>(); fa.cwT = cwT;
+      fa.cs64 = p->cs64.as<double>(); fa.csT = csT; fa.resid_out = ctx->resid.as<double>();
+      fa.flags = flags;
+      k_badmm_finalize<T><<<K, NT, sizeof(double) * (R + 8), S>>>(fa);
+      D2MI_HIP(hipGetLastError());
+      ++n_launch;
+    }
+  } else {
+    // ---- plain form
+    const size_t n = (size_t)s * p->col;
+    DevMem &Xm = F64 ? ctx->X64 : ctx->X32;
+    D2MI_TRY(Xm.ensure(n * sizeof(T)));
+    const size_t accn = (size_t)K * s * (d + 2);
+    D2MI_TRY(ctx->rec.ensure(sizeof(double) * accn));
+    k_zero_f64<<<256, NT, 0, S>>>(ctx->rec.as<double>(), accn);
+    evs.resize(2 * (maxIters + 1));
+    for (auto &e : evs) D2MI_HIP(hipEventCreate(&e));
+    for (int iter = 0; iter <= maxIters; ++iter) {
+      const int upd = euclid_upd && (iter % updatePerLoops == 0);
+      PlainArgs<T> pa{};
+      pa.N = N; pa.s = s; pa.d = d; pa.V = p->V; pa.upd = upd;
+      pa.p_str = p->p_str.as<int>(); pa.cum = p->cum.as<long long>(); pa.label = ctx->label.as<int>();
+      pa.w = w; pa.supp = supp; pa.sym = sym; pa.dist = dist; pa.cw = cwT; pa.csupp = csT;
+      pa.scal = scal; pa.Z = Z; pa.Y = Y; pa.X = (T *)Xm.p; pa.acc = ctx->rec.as<double>();
+      D2MI_HIP(hipEventRecord(evs[2 * iter], S));
+      k_badmm_plain<T, CM><<<(N + 127) / 128, 128, 0, S>>>(pa);
+      D2MI_HIP(hipEventRecord(evs[2 * iter + 1], S));
+      k_badmm_plain_finalize<T><<<K, NT, 0, S>>>(K, s, d, upd, ctx->rec.as<double>(),
+                                                 p->cw64.as<double>(), cwT,
+                                                 p->cs64.as<double>(), csT, flags);
+      D2MI_HIP(hipGetLastError());
+      ++n_launch;
+    }
+  }
+  D2MI_HIP(hipEventRecord(ctx->ev[1], S));
+  D2MI_HIP(hipStreamSynchronize(S));
+  for (int i = 0; i < n_launch; ++i) {
+    float ms = 0.f;
+    D2MI_HIP(hipEventElapsedTime(&ms, evs[2 * i], evs[2 * i + 1]));
+    ms_plan += ms;
+  }
+  for (auto &e : evs) (void)hipEventDestroy(e);
+  D2MI_HIP(hipMemcpy(&hflags, flags, sizeof(int), hipMemcpyDeviceToHost));
+  if (hflags & FLAG_NUMERIC) {
+    d2mi_set_error("a normaliser was not positive (blas_like64.c:131/148 assert)");
+    return D2MI_ENUMERIC;
+  }
+  double hscal[2];
+  D2MI_HIP(hipMemcpy(hscal, scal, sizeof(hscal), hipMemcpyDeviceToHost));
+  if (st) {
+    float ms = 0.f;
+    D2MI_HIP(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+    st->rho = hscal[0];
+    st->ms_total = ms;
+    st->ms_plan = ms_plan;
+    st->n_plan_launches = n_launch;
+    st->plan_elems = (double)s * (double)p->col;
+    st->n_trace = 0;
+    st->obj = 0.0;
+    if (variant == D2MI_BADMM_ENGINE) {
+      std::vector<double> res(3 * (size_t)(maxIters + 1));
+      D2MI_HIP(hipMemcpy(res.data(), ctx->resid.p, sizeof(double) * res.size(),
+                         hipMemcpyDeviceToHost));
+      const double Nn = (double)N;
+      for (int iter = 0; iter <= maxIters; ++iter) {
+        const bool ref_row = (iter % 100 == 99) || (iter < 100 && iter % 20 == 19);
+        if (ref_row && st->n_trace < 64) {
+          st->trace[st->n_trace][0] = iter + 1;
+          st->trace[st->n_trace][1] = res[3 * iter] * hscal[0] / Nn;
+          st->trace[st->n_trace][2] = res[3 * iter + 1] / Nn;
+          st->trace[st->n_trace][3] = res[3 * iter + 2] / Nn;
+          st->n_trace++;
+        }
+      }
+      st->obj = res[3 * maxIters] * hscal[0] / Nn;
+    }
+  }
+  return D2MI_OK;
+}
+
+}  // namespace
+
+extern "C" int d2mi_badmm(d2mi_ctx *ctx, int phase, int maxIters, double rhoCoeff,
+                          int updatePerLoops, int precision, int variant,
+                          d2mi_badmm_stats *stats) {
+  D2MI_TRY(d2mi_check_ctx(ctx));
+  PhaseDev *p = d2mi_phase(ctx, phase);
+  if (!p) return D2MI_EINVAL;
+  if (!p->K || !ctx->label.p) {
+    d2mi_set_error("set centroids and labels before d2mi_badmm");
+    return D2MI_EINVAL;
+  }
+  if (maxIters < 0 || updatePerLoops <= 0 || !(rhoCoeff > 0) ||
+      (precision != 32 && precision != 64) ||
+      (variant != D2MI_BADMM_ENGINE && variant != D2MI_BADMM_PLAIN)) {
+    d2mi_set_error("d2mi_badmm: bad options");
+    return D2MI_EINVAL;
+  }
+  if (p->metric == D2MI_WORD_EMBED) {
+    d2mi_set_error("word-embedding B-ADMM is not built yet");
+    return D2MI_EUNSUPPORTED;
+  }
+  if (p->metric == D2MI_EUCLIDEAN && p->dim > MAXD_FUSED) {
+    d2mi_set_error("Euclidean B-ADMM handles d <= %d in this build", MAXD_FUSED);
+    return D2MI_EUNSUPPORTED;
+  }
+  if (p->K > 16384) {
+    d2mi_set_error("K = %d too large", p->K);
+    return D2MI_EUNSUPPORTED;
+  }
+  const int fresh = d2mi_ensure_plans(ctx, p, precision);
+  if (fresh < 0) return fresh;
+  const bool f = fresh == 1;
+  if (precision == 64) {
+    switch (p->metric) {
+      case D2MI_EUCLIDEAN:
+        return run_badmm<double, COST_EUCLID>(ctx, p, maxIters, rhoCoeff, updatePerLoops, variant, stats, f);
+      case D2MI_HISTOGRAM:
+        return run_badmm<double, COST_HIST>(ctx, p, maxIters, rhoCoeff, updatePerLoops, variant, stats, f);
+      default:
+        return run_badmm<double, COST_SPARSE>(ctx, p, maxIters, rhoCoeff, updatePerLoops, variant, stats, f);
+    }
+  }
+  switch (p->metric) {
+    case D2MI_EUCLIDEAN:
+      return run_badmm<float, COST_EUCLID>(ctx, p, maxIters, rhoCoeff, updatePerLoops, variant, stats, f);
+    case D2MI_HISTOGRAM:
+      return run_badmm<float, COST_HIST>(ctx, p, maxIters, rhoCoeff, updatePerLoops, variant, stats, f);
+    default:
+      return run_badmm<float, COST_SPARSE>(ctx, p, maxIters, rhoCoeff, updatePerLoops, variant, stats, f);
+  }
+}

@@ -1,0 +1,78 @@
+// Internal declarations shared by the libd2mi.so translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstddef>
+#include <cstdint>
+#include <vector>
+
+#include "d2mi.h"
+
+void d2mi_set_error(const char *fmt, ...);
+
+#define D2MI_HIP(x)                                                                   \
+  do {                                                                                \
+    hipError_t e_ = (x);                                                              \
+    if (e_ != hipSuccess) {                                                           \
+      d2mi_set_error("%s:%d: %s failed: %s", __FILE__, __LINE__, #x, hipGetErrorString(e_)); \
+      return D2MI_EHIP;                                                               \
+    }                                                                                 \
+  } while (0)
+
+#define D2MI_TRY(x)        \
+  do {                     \
+    int r_ = (x);          \
+    if (r_ != D2MI_OK) return r_; \
+  } while (0)
+
+// Device allocation that remembers its size and can grow.
+struct DevMem {
+  void *p = nullptr;
+  size_t bytes = 0;
+  int ensure(size_t want);  // grows (contents not kept) to at least `want` bytes
+  int assign(const void *host, size_t nbytes, hipStream_t s);  // ensure + H2D copy
+  void release();
+  template <typename T> T *as() const { return static_cast<T *>(p); }
+};
+
+// One phase of the data set resident in HBM (the reference's `sph`,
+// include/d2/clustering.h:14-95) plus its centroids and B-ADMM state.
+struct PhaseDev {
+  bool set = false;
+  int metric = 0, dim = 0, V = 0, m_max = 0;
+  size_t N = 0, col = 0;
+  DevMem p_str;                 // int[N]
+  DevMem cum;                   // int64[N]
+  DevMem w64, w32;              // weights  double/float [col]
+  DevMem supp64, supp32;        // supports double/float [col*dim] (Euclidean)
+  DevMem sym;                   // int[col] word ids / histogram bins
+  DevMem vocab64, vocab32;      // [V*dim]
+  DevMem dist64, dist32;        // [V*V]
+  // centroids (K x s)
+  int K = 0, s = 0;
+  DevMem cw64, cw32;            // [K*s]
+  DevMem cs64, cs32;            // [K*s*dim]
+  // resident transport plans, element (k, j) of object i at s*cum[i] + k + j*s
+  DevMem Z32, Y32, Z64, Y64;
+  int plan_s32 = 0, plan_s64 = 0;  // s the plans were sized for (0 = none)
+};
+
+struct d2mi_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  size_t N = 0;
+  DevMem label, lsw;            // int[N], char[N]
+  std::vector<PhaseDev> ph;
+  // B-ADMM scratch
+  DevMem perm, blkcnt, segstart, rec, part, scal, flags, resid, X32, X64;
+  // pdist2 scratch
+  DevMem pA, pB, pC;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  size_t lds_max = 0;
+  int num_cu = 0;
+};
+
+int d2mi_check_ctx(d2mi_ctx *ctx);
+PhaseDev *d2mi_phase(d2mi_ctx *ctx, int phase);

@@ -1,0 +1,111 @@
+/*
+ * d2mi.h -- C ABI of libd2mi.so, the MI355X (gfx950) engine for the
+ * AD2-clustering hot path of bobye/d2_kmeans.
+ *
+ * Plain pointers and sizes only.  Host arrays use the reference's layouts
+ * (include/d2/clustering.h:14-124 with SCALAR = double): per phase, CSR-style
+ * ragged objects (p_str[i] supports starting at column p_str_cum[i]), Euclidean
+ * supports row-major per support (p_supp[col*dim]), word ids / histogram bins
+ * 0-based in p_supp_sym[col]; centroids are K objects of exactly s supports
+ * (centroid_rand.c:177-183): weights p_w[K*s], supports p_supp[K*s*dim].
+ *
+ * Every function returns 0 on success and a negative D2MI_E* code on error;
+ * d2mi_last_error() then returns a message.  A context owns one HIP stream
+ * on one device; calls on one context must not overlap.
+ */
+#ifndef D2MI_H
+#define D2MI_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define D2MI_OK 0
+#define D2MI_EINVAL -1      /* bad argument */
+#define D2MI_EHIP -2        /* HIP runtime error */
+#define D2MI_ENOMEM -3      /* device allocation failed */
+#define D2MI_EEMPTY -4      /* a cluster has no object (centroid_Bregman.c:144) */
+#define D2MI_ENUMERIC -5    /* a normaliser was not positive (blas_like64.c:131,148) */
+#define D2MI_EUNSUPPORTED -6 /* shape or metric this build does not handle */
+
+/* metric ids, include/d2/param.h:23-28 */
+#define D2MI_EUCLIDEAN 0
+#define D2MI_HISTOGRAM 5
+#define D2MI_WORD_EMBED 7
+#define D2MI_SPARSE_HISTOGRAM 12
+
+/* B-ADMM kernel variants */
+#define D2MI_BADMM_ENGINE 0 /* LDS-staged workgroup-per-object kernels */
+#define D2MI_BADMM_PLAIN 1  /* one thread per object (comparison baseline) */
+
+typedef struct d2mi_ctx d2mi_ctx;
+
+const char *d2mi_last_error(void);
+int d2mi_device_count(int *count);
+
+/* Context on `device`.  The reference keeps this state in d2_clustering's
+ * locals and var_mph (clustering.c:421-449, clustering_util.c:152-230). */
+int d2mi_create(d2mi_ctx **out, int device);
+int d2mi_destroy(d2mi_ctx *ctx);
+int d2mi_synchronize(d2mi_ctx *ctx);
+
+/* Uploads one phase of the data set (the `sph` filled by d2_read,
+ * clustering_io.c:14-202).  Arrays are copied; the caller keeps ownership.
+ * Unused arrays may be NULL (p_supp for symbolic metrics, etc.).  Replaces
+ * any earlier upload of that phase and discards its B-ADMM state. */
+int d2mi_set_phase(d2mi_ctx *ctx, int phase, int metric, int dim, size_t size, size_t col,
+                   const int *p_str, const size_t *p_str_cum, const double *p_w,
+                   const double *p_supp, const int *p_supp_sym, int vocab_size,
+                   const double *vocab_vec, const double *dist_mat);
+
+/* Labels (int[N], -1 = none) and label-switch flags (char[N]) shared by all
+ * phases (mph.label, var_mph.label_switch; clustering.h:121, :198). */
+int d2mi_set_labels(d2mi_ctx *ctx, const int *label, const char *label_switch);
+int d2mi_get_labels(d2mi_ctx *ctx, int *label, char *label_switch);
+
+/* Centroids of one phase: K x s supports (the `sph` centroids of
+ * d2_init_centroid, clustering.c:379-404). p_supp may be NULL for
+ * histogram metrics. */
+int d2mi_set_centroids(d2mi_ctx *ctx, int phase, int K, int s, const double *p_w,
+                       const double *p_supp);
+int d2mi_get_centroids(d2mi_ctx *ctx, int phase, double *p_w, double *p_supp);
+
+typedef struct {
+  double rho;          /* rho chosen at setup (centroid_Bregman.c:109) */
+  double obj;          /* rho * <C, X> / N at loop exit */
+  int n_trace;         /* residual rows recorded (centroid_Bregman.c:300-316) */
+  double trace[64][4]; /* (iter+1, obj, primres, dualres) */
+  double ms_total;     /* device time of the whole update (HIP events) */
+  double ms_plan;      /* summed device time of the plan-update kernel */
+  int n_plan_launches; /* launches of the plan-update kernel */
+  double plan_elems;   /* s * col, plan elements per inner iteration */
+} d2mi_badmm_stats;
+
+/* One B-ADMM centroid update of `phase` (d2_centroid_sphBregman,
+ * centroid_Bregman.c:51-325, warm start c0 == c): centroids are updated in
+ * place on the device, the transport plan Z stays resident across calls
+ * (warm start) and is re-initialised for objects whose label switched.
+ * precision: 32 (FP32 plans, FP64 reductions) or 64 (all FP64).
+ * variant: D2MI_BADMM_ENGINE or D2MI_BADMM_PLAIN.  stats may be NULL. */
+int d2mi_badmm(d2mi_ctx *ctx, int phase, int maxIters, double rhoCoeff, int updatePerLoops,
+               int precision, int variant, d2mi_badmm_stats *stats);
+
+/* Copies the resident plan (which: 0 = Z, 1 = Y) of `phase` at `precision`
+ * to host memory as double[s*col]. */
+int d2mi_get_plan(d2mi_ctx *ctx, int phase, int precision, int which, double *out);
+/* Sets the resident plan Z (double[s*col]) at `precision`. */
+int d2mi_set_plan(d2mi_ctx *ctx, int phase, int precision, const double *Z);
+
+/* Squared Euclidean distance matrix, the reference's _dpdist2
+ * (blas_like64.c:199-208, include/utils/blas_like.h:48):
+ * C[i*n + j] = sum_k (A[j*d + k] - B[i*d + k])^2, A: n x d, B: m x d,
+ * host arrays, computed on the device in FP64. */
+int d2mi_pdist2(d2mi_ctx *ctx, int d, size_t n, size_t m, const double *A, const double *B,
+                double *C);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,20 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs through libd2mi.so)")
+
+
+@pytest.fixture(scope="session")
+def engine():
+    from d2_kmeans_amd.engine import Engine
+    e = Engine(0)
+    yield e
+    e.close()

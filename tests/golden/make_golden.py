@@ -1,0 +1,148 @@
+"""Writes the golden fixtures of tests/golden/ (run in a container that has
+/root/reference; the GPU box never runs this).
+
+* ref_badmm.npz   -- the reference's own d2_centroid_sphBregman
+                     (centroid_Bregman.c:51-325, built from its sources by
+                     oracle/Makefile.ref) on a seeded 3-d Gaussian-mixture
+                     case: a cold call (every label switched) and a warm
+                     call (a third of the labels switched), maxIters = 10.
+* ref_pdist2.npz  -- the reference's _dpdist2 (blas_like64.c:199-208).
+* ref_rands.npz   -- the reference's d2_centroid_rands (centroid_rand.c:161)
+                     with its glibc-rand shuffle (math.c:39-56), K=8, s=6.
+* lp_highs.npz    -- exact transport-LP values of the problem the reference
+                     poses to Mosek (solver_mosek.cc:61-143: row sums = wX,
+                     column sums = wY except the last), solved by scipy's
+                     HiGHS, an independent exact LP solver.
+* mountaindat_head.d2 -- the first 300 objects of the reference's
+                     data/mountaindat.d2 (2 phases, d = 3), an input file.
+
+    python tests/golden/make_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from d2_kmeans_amd import synth  # noqa: E402
+from d2_kmeans_amd.data import BADMMOptions, Centroids  # noqa: E402
+from oracle import oracle as O  # noqa: E402
+
+
+def phase_arrays(ph):
+    return dict(p_str=ph.p_str, p_w=ph.p_w, p_supp=ph.p_supp, dim=ph.dim)
+
+
+def make_badmm():
+    data, _ = synth.gaussian_mixture(n=240, seed=7)
+    ph = data.phases[0]
+    K, s = 4, 5
+    rng = np.random.default_rng(11)
+    c0 = O.ref_centroid_rands(ph, K, s)
+    label = rng.integers(0, K, size=ph.size).astype(np.int32)
+    lsw = np.ones(ph.size, np.int8)
+    opts = BADMMOptions(10, 2.0, 10)
+    c1 = c0.copy()
+    Z = np.zeros(s * ph.col)
+    X1, Y1, C1 = O.ref_badmm(ph, label, lsw, c1, Z, opts)
+    Z1 = Z.copy()
+    # warm call: a third of the objects move to another label
+    label2 = label.copy()
+    mv = rng.random(ph.size) < 1 / 3
+    label2[mv] = (label2[mv] + 1) % K
+    lsw2 = mv.astype(np.int8)
+    c2 = c1.copy()
+    X2, Y2, C2 = O.ref_badmm(ph, label2, lsw2, c2, Z, opts)
+    np.savez_compressed(os.path.join(HERE, "ref_badmm.npz"), **phase_arrays(ph), K=K, s=s,
+             maxIters=opts.maxIters, rhoCoeff=opts.rhoCoeff, updatePerLoops=opts.updatePerLoops,
+             c0_w=c0.p_w, c0_supp=c0.p_supp, label=label, lsw=lsw,
+             c1_w=c1.p_w, c1_supp=c1.p_supp, Z1=Z1, X1=X1, Y1=Y1,
+             label2=label2, lsw2=lsw2, c2_w=c2.p_w, c2_supp=c2.p_supp, Z2=Z, X2=X2, Y2=Y2)
+
+
+def make_pdist2():
+    rng = np.random.default_rng(3)
+    A = rng.standard_normal((7, 5))
+    B = rng.standard_normal((11, 5))
+    np.savez_compressed(os.path.join(HERE, "ref_pdist2.npz"), A=A, B=B, C=O.ref_pdist2(A, B))
+
+
+def make_rands():
+    data, _ = synth.gaussian_mixture()
+    c = O.ref_centroid_rands(data.phases[0], 8, 6)
+    np.savez_compressed(os.path.join(HERE, "ref_rands.npz"), p_w=c.p_w, p_supp=c.p_supp)
+
+
+def lp_highs(Cm, wX, wY):
+    from scipy.optimize import linprog
+    n, m = len(wX), len(wY)
+    A = []
+    b = []
+    for i in range(n):  # row sums
+        row = np.zeros(n * m)
+        row[i + np.arange(m) * n] = 1
+        A.append(row)
+        b.append(wX[i])
+    for j in range(m - 1):  # column sums, last dropped
+        row = np.zeros(n * m)
+        row[np.arange(n) + j * n] = 1
+        A.append(row)
+        b.append(wY[j])
+    r = linprog(Cm, A_eq=np.array(A), b_eq=np.array(b), bounds=(0, None), method="highs",
+                options=dict(primal_feasibility_tolerance=1e-10, dual_feasibility_tolerance=1e-10))
+    assert r.status == 0
+    return r.fun
+
+
+def make_lp():
+    rng = np.random.default_rng(5)
+    shapes = [(1, 1), (1, 5), (5, 1), (2, 2), (3, 3)] + \
+        [tuple(rng.integers(1, 13, 2)) for _ in range(120)] + [(40, 60), (64, 100), (6, 12)]
+    out = dict(n=[], m=[], C=[], wX=[], wY=[], val=[])
+    for n, m in shapes:
+        n, m = int(n), int(m)
+        pts_a = rng.standard_normal((n, 3))
+        pts_b = rng.standard_normal((m, 3))
+        Cm = ((pts_a[:, None, :] - pts_b[None, :, :]) ** 2).sum(-1).T.reshape(-1)  # [i + j*n]
+        wX = rng.gamma(1.0, size=n)
+        wX /= wX.sum()
+        wY = rng.gamma(1.0, size=m)
+        wY /= wY.sum()
+        out["n"].append(n)
+        out["m"].append(m)
+        out["C"].append(Cm)
+        out["wX"].append(wX)
+        out["wY"].append(wY)
+        out["val"].append(lp_highs(Cm, wX, wY))
+    np.savez_compressed(os.path.join(HERE, "lp_highs.npz"), n=np.array(out["n"]), m=np.array(out["m"]),
+             C=np.concatenate(out["C"]), wX=np.concatenate(out["wX"]),
+             wY=np.concatenate(out["wY"]), val=np.array(out["val"]))
+
+
+def make_mountaindat():
+    src = "/root/reference/data/mountaindat.d2"
+    lines = open(src).read().split("\n")
+    # 2 phases per object, each phase: dim, n, weights, n support lines
+    out, i, objs = [], 0, 0
+    while objs < 300:
+        for _ in range(2):
+            d = int(lines[i]); n = int(lines[i + 1])
+            out.extend(lines[i:i + 3 + n])
+            i += 3 + n
+        objs += 1
+    with open(os.path.join(HERE, "mountaindat_head.d2"), "w") as f:
+        f.write("\n".join(out) + "\n")
+
+
+if __name__ == "__main__":
+    O.build()
+    assert O.build_ref(), "needs /root/reference"
+    make_badmm()
+    make_pdist2()
+    make_rands()
+    make_lp()
+    make_mountaindat()
+    print("golden fixtures written to", HERE)
