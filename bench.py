@@ -1,0 +1,202 @@
+"""B-ADMM plan-update throughput on one MI355X (the image-like D2 workload).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Workload: one Euclidean phase, d=2, supports on the 16x16 integer grid,
+N=200k objects with U{20..80} distinct cells (10 stroke templates, 10%
+jitter), intensities U(0,1] normalised; K=10 centroids of s=40 supports;
+B-ADMM with maxIters=10 (11 inner passes, iterations 0..10), rho coefficient
+2.0, supports refreshed every 10 passes.  A step is one whole centroid update
+(setup + 11 passes), the way d2_clustering calls it once per outer
+iteration; inputs are resident in HBM before timing starts.
+
+Metric: plan updates/s = N * 11 / seconds per step (one plan update = one
+inner pass over one object's s x m_i plan).  It counts only when the FP32
+objective stays within 1e-5 of an FP64 run from the same state
+(``gate_obj_rel_err``).  Prints ONE JSON line on stdout; progress goes to
+stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "B-ADMM plan updates/s at obj rel err ≤1e-5 vs FP64; assign p50/p99 ms"
+HBM_PEAK_GBPS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+class _Quiet:
+    """Silences C-level stdout (the reference build prints its tables)."""
+
+    def __enter__(self):
+        sys.stdout.flush()
+        self.fd = os.dup(1)
+        self.null = os.open(os.devnull, os.O_WRONLY)
+        os.dup2(self.null, 1)
+
+    def __exit__(self, *a):
+        os.dup2(self.fd, 1)
+        os.close(self.fd)
+        os.close(self.null)
+
+
+def cpu_baseline(data, labels, c0, opts, n_sample):
+    """The reference's own B-ADMM (oracle/_ref, built from its sources,
+    serial FP64, MKL sequential) on the first n_sample objects of the same
+    workload; falls back to this project's FP64 restatement."""
+    from oracle import oracle as O
+    sub = data.subset(np.arange(n_sample))
+    ph = sub.phases[0]
+    lab = labels[:n_sample].copy()
+    lsw = np.ones(n_sample, np.int8)
+    kind = "reference" if O.ref_available() else "port"
+    c = c0.copy()
+    Z = np.zeros(c.s * ph.col)
+    t0 = time.perf_counter()
+    if kind == "reference":
+        with _Quiet():
+            O.ref_badmm(ph, lab, lsw, c, Z, opts)
+    else:
+        O.badmm(ph, lab, lsw, c, Z, opts)
+    dt = time.perf_counter() - t0
+    return {"value": n_sample * (opts.maxIters + 1) / dt, "unit": "plan updates/s", "cores": 1,
+            "kind": kind, "seconds": dt,
+            "sample": f"first {n_sample} objects of the same workload (s=40, K=10), one "
+                      f"{opts.maxIters + 1}-pass B-ADMM update, serial FP64"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=200_000)
+    ap.add_argument("--cpu-sample", type=int, default=6000)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-gate", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(see the sharding design); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from d2_kmeans_amd import synth
+    from d2_kmeans_amd.data import BADMMOptions
+    from d2_kmeans_amd.engine import Engine
+
+    K, s = 10, 40
+    opts = BADMMOptions(10, 2.0, 10)
+    log(f"[bench] generating grid-strokes workload N={args.n}")
+    data, meta = synth.grid_strokes(n=args.n)
+    labels = meta["classes"]
+    ph = data.phases[0]
+    c0 = synth.seed_centroids(ph, K, s, labels=labels)
+    eng = Engine(0)
+    eng.set_data(data)
+
+    def reset(precision, lsw_all=True):
+        eng.set_centroids(0, c0)
+        eng.set_labels(labels, np.ones(ph.size, np.int8) if lsw_all else np.zeros(ph.size, np.int8))
+
+    # ---- gate: FP32 vs FP64 from the same state
+    gate = None
+    if not args.no_gate:
+        log("[bench] gate: FP64 and FP32 updates from the same state")
+        reset(64)
+        st64 = eng.badmm(0, opts, precision=64)
+        c64 = eng.get_centroids(0)
+        reset(32)
+        st32 = eng.badmm(0, opts, precision=32)
+        c32 = eng.get_centroids(0)
+        gate = {
+            "obj_fp64": st64["obj"], "obj_fp32": st32["obj"],
+            "obj_rel_err": abs(st32["obj"] - st64["obj"]) / abs(st64["obj"]),
+            "centroid_supp_max_rel": float(np.max(np.abs(c32.p_supp - c64.p_supp)) / np.max(np.abs(c64.p_supp))),
+            "centroid_w_max_abs": float(np.max(np.abs(c32.p_w - c64.p_w))),
+        }
+        log(f"[bench] gate: {gate}")
+
+    # ---- timed engine steps (FP32 plans, FP64 reductions)
+    reset(32)
+    for i in range(args.warmup):
+        eng.badmm(0, opts, precision=32)
+        log(f"[bench] warmup {i} done")
+    eng.set_labels(labels, np.zeros(ph.size, np.int8))  # steady state: no label switches
+    eng.synchronize()
+    plan_ms, plan_launches, totals = 0.0, 0, []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        st = eng.badmm(0, opts, precision=32)
+        plan_ms += st["ms_plan"]
+        plan_launches += st["n_plan_launches"]
+        totals.append(st["ms_total"])
+    eng.synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = 1000.0 * wall / args.steps
+    passes = opts.maxIters + 1
+    value = ph.size * passes / (ms_per_step / 1000.0)
+    log(f"[bench] {args.steps} steps: {ms_per_step:.3f} ms/step, device {np.mean(totals):.3f} ms")
+
+    elems = st["plan_elems"]
+    avg_plan_ms = plan_ms / plan_launches
+    bytes_per_launch = 16.0 * elems  # Z and Y read + written once, FP32
+    achieved = bytes_per_launch / (avg_plan_ms / 1000.0) / 1e9
+
+    # ---- plain one-thread-per-object form (the 20x denominator)
+    plain = None
+    if not args.no_plain:
+        reset(32)
+        eng.badmm(0, opts, precision=32, variant="plain")
+        eng.set_labels(labels, np.zeros(ph.size, np.int8))
+        t1 = time.perf_counter()
+        stp = eng.badmm(0, opts, precision=32, variant="plain")
+        eng.synchronize()
+        pw = time.perf_counter() - t1
+        plain = {"value": ph.size * passes / pw, "ms_per_step": 1000 * pw,
+                 "plan_kernel_ms_avg": stp["ms_plan"] / stp["n_plan_launches"]}
+        log(f"[bench] plain form: {plain}")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log(f"[bench] cpu baseline on {args.cpu_sample} objects")
+        cpu = cpu_baseline(data, labels, c0, opts, args.cpu_sample)
+        log(f"[bench] cpu baseline: {cpu}")
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "plan updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": "grid-strokes D2, 1 Euclidean phase d=2, 16x16 grid, "
+                               f"N={ph.size}, m in [20,80], K={K}, s={s}, B-ADMM maxIters=10 "
+                               "(11 passes), rhoCoeff=2.0, supports every 10 passes",
+                   "objects": ph.size, "plan_elements": int(elems), "K": K, "s": s},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+                     "kernel": "k_badmm_plan<float,EUCLID>",
+                     "kernel_ms_avg": avg_plan_ms, "algorithmic_bytes_per_launch": bytes_per_launch},
+        "gate_obj_rel_err": None if gate is None else gate["obj_rel_err"],
+        "gate": gate,
+        "plain_form": plain,
+        "speedup_vs_plain": None if plain is None else value / plain["value"],
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    eng.close()
+
+
+if __name__ == "__main__":
+    main()

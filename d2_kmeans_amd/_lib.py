@@ -31,6 +31,17 @@ class BadmmStats(C.Structure):
                 ("plan_elems", C.c_double)]
 
 
+class LabelStats(C.Structure):
+    _fields_ = [("changes", C.c_ulonglong), ("dist_count", C.c_ulonglong), ("cost", C.c_double),
+                ("ms", C.c_double)]
+
+
+class ClusterStats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("changes", C.c_longlong * 256),
+                ("dist_count", C.c_longlong * 256), ("final_cost", C.c_double),
+                ("ms_labeling", C.c_double), ("ms_badmm", C.c_double)]
+
+
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
 _sp = C.POINTER(C.c_size_t)
@@ -55,6 +66,14 @@ SIGNATURES = {
     "d2mi_get_plan": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _dp]),
     "d2mi_set_plan": (C.c_int, [_vp, C.c_int, C.c_int, _dp]),
     "d2mi_pdist2": (C.c_int, [_vp, C.c_int, C.c_size_t, C.c_size_t, _dp, _dp, _dp]),
+    "d2mi_transport_batch": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _ip]),
+    "d2mi_distances": (C.c_int, [_vp, C.c_int, C.c_size_t, _ip, _ip, _dp]),
+    "d2mi_elkan_reset": (C.c_int, [_vp, C.c_int]),
+    "d2mi_labeling": (C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(LabelStats)]),
+    "d2mi_save_centroids": (C.c_int, [_vp, C.c_int]),
+    "d2mi_elkan_post": (C.c_int, [_vp, C.c_int]),
+    "d2mi_clustering": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+                                  C.c_int, C.c_int, C.POINTER(ClusterStats)]),
 }
 
 _lib = None

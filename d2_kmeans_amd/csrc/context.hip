@@ -96,7 +96,7 @@ extern "C" int d2mi_create(d2mi_ctx **out, int device) {
 static void release_phase(PhaseDev &p) {
   DevMem *all[] = {&p.p_str, &p.cum, &p.w64, &p.w32, &p.supp64, &p.supp32, &p.sym,
                    &p.vocab64, &p.vocab32, &p.dist64, &p.dist32, &p.cw64, &p.cw32,
-                   &p.cs64, &p.cs32, &p.Z32, &p.Y32, &p.Z64, &p.Y64};
+                   &p.cs64, &p.cs32, &p.Z32, &p.Y32, &p.Z64, &p.Y64, &p.cw_old, &p.cs_old};
   for (DevMem *m : all) m->release();
   p = PhaseDev();
 }
@@ -108,7 +108,9 @@ extern "C" int d2mi_destroy(d2mi_ctx *ctx) {
   for (auto &p : ctx->ph) release_phase(p);
   DevMem *all[] = {&ctx->label, &ctx->lsw, &ctx->perm, &ctx->blkcnt, &ctx->segstart,
                    &ctx->rec, &ctx->part, &ctx->scal, &ctx->flags, &ctx->resid,
-                   &ctx->X32, &ctx->X64, &ctx->pA, &ctx->pB, &ctx->pC};
+                   &ctx->X32, &ctx->X64, &ctx->pA, &ctx->pB, &ctx->pC, &ctx->el_l,
+                   &ctx->el_u, &ctx->el_s, &ctx->el_c, &ctx->el_r, &ctx->dist, &ctx->mask,
+                   &ctx->jobs, &ctx->lpstat, &ctx->cnt, &ctx->mind2, &ctx->red};
   for (DevMem *m : all) m->release();
   for (auto &e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
@@ -354,7 +356,8 @@ extern "C" int d2mi_set_plan(d2mi_ctx *ctx, int phase, int precision, const doub
     if (p) d2mi_set_error("set centroids before the plan");
     return D2MI_EINVAL;
   }
-  D2MI_TRY(d2mi_ensure_plans(ctx, p, precision));
+  const int fresh = d2mi_ensure_plans(ctx, p, precision);
+  if (fresh < 0) return fresh;
   const size_t n = (size_t)p->s * p->col;
   if (precision == 64) {
     D2MI_HIP(hipMemcpyAsync(p->Z64.p, Z, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));

@@ -54,6 +54,7 @@ struct PhaseDev {
   int K = 0, s = 0;
   DevMem cw64, cw32;            // [K*s]
   DevMem cs64, cs32;            // [K*s*dim]
+  DevMem cw_old, cs_old;        // FP64 copy taken before a centroid update (Elkan)
   // resident transport plans, element (k, j) of object i at s*cum[i] + k + j*s
   DevMem Z32, Y32, Z64, Y64;
   int plan_s32 = 0, plan_s64 = 0;  // s the plans were sized for (0 = none)
@@ -69,6 +70,10 @@ struct d2mi_ctx {
   DevMem perm, blkcnt, segstart, rec, part, scal, flags, resid, X32, X64;
   // pdist2 scratch
   DevMem pA, pB, pC;
+  // labeling / Elkan state (clustering.h:106-112), resident in HBM
+  int elkan_K = 0;
+  DevMem el_l, el_u, el_s, el_c, el_r;  // double[N*K], double[N], double[K], double[K*K], char[N]
+  DevMem dist, mask, jobs, lpstat, cnt, mind2, red;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   size_t lds_max = 0;
   int num_cu = 0;

@@ -1,0 +1,51 @@
+// Batched exact transport LPs on the device (transport.hip).
+#pragma once
+
+#include "d2mi_internal.h"
+
+enum LpCostMode {
+  LP_COST_EUCLID = 0,  // squared Euclidean distance of supports held in LDS
+  LP_COST_HIST = 1,    // dense histogram ground cost dist[k + j*V]
+  LP_COST_SPARSE = 2,  // sparse histogram: dist[id_j * V + k]
+  LP_COST_GLOBAL = 3,  // precomputed costs gcost[job*gstride + k + j*n]
+};
+
+// One LP per job: rows = the sA supports of entry job_a[job] of centroid
+// table A, columns = object job_b[job] of a phase (b_is_table == 0) or the sB
+// supports of entry job_b[job] of centroid table B.  The LP value is ADDED
+// to out[job] (phases accumulate in call order).
+struct LpArgs {
+  int njobs;
+  const int *job_a;
+  const int *job_b;
+  int implicit_K;              // > 0: job -> (job_b = job / K, job_a = job % K), no job arrays
+  const unsigned char *mask;   // optional: jobs with mask[job] == 0 are skipped
+  int b_is_table;
+  int cost_mode;
+  int d;
+  int sA;
+  const double *A_supp;
+  const double *A_w;
+  const int *p_str;
+  const long long *cum;
+  const double *w;
+  const double *supp;
+  const int *sym;
+  int sB;
+  const double *B_supp;
+  const double *B_w;
+  int V;
+  const double *dist;
+  const double *gcost;
+  long long gstride;
+  int n_max, m_max;
+  double *out;
+  int *pivots;  // may be null
+  int *status;  // bit 0: an LP hit the pivot limit
+};
+
+size_t d2mi_lp_lds_bytes(int n_max, int m_max, int dsupp);
+int d2mi_launch_transport(d2mi_ctx *ctx, const LpArgs &a);
+int d2mi_zero_doubles(d2mi_ctx *ctx, double *p, int n);
+int d2mi_zero_ints(d2mi_ctx *ctx, int *p, int n);
+int d2mi_sqrt_clamp(d2mi_ctx *ctx, double *p, int n);

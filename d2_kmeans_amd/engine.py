@@ -135,3 +135,59 @@ class Engine:
         out = np.empty(n * m)
         check(lib().d2mi_pdist2(self._h, d, n, m, ptr(A), ptr(B), ptr(out)))
         return out
+
+    # ------------------------------------------------------- transport LP
+    def transport_batch(self, C, wX, wY, pivots=False):
+        """Exact LP values of a batch of same-shape transport problems
+        (d2_match_by_distmat's LP, solver_mosek.cc:61-143).  C: (B, n*m) with
+        element (i, j) at [i + j*n]; wX: (B, n); wY: (B, m)."""
+        wX = np.ascontiguousarray(np.atleast_2d(wX), np.float64)
+        wY = np.ascontiguousarray(np.atleast_2d(wY), np.float64)
+        nb, n = wX.shape
+        m = wY.shape[1]
+        C = np.ascontiguousarray(C, np.float64).reshape(nb, n * m)
+        out = np.empty(nb)
+        piv = np.empty(nb, np.int32)
+        check(lib().d2mi_transport_batch(self._h, n, m, nb, ptr(C), ptr(wX), ptr(wY), ptr(out),
+                                         ptr(piv, _lib._ip)))
+        return (out, piv) if pivots else out
+
+    # -------------------------------------------------------- assignment
+    def distances(self, obj, cent, selected_phase: int = -1) -> np.ndarray:
+        """Batched d2_compute_distance (clustering.c:33-128): exact
+        sqrt(sum over phases of the transport-LP value) per (object, centroid)."""
+        obj = np.ascontiguousarray(obj, np.int32)
+        cent = np.ascontiguousarray(cent, np.int32)
+        out = np.empty(len(obj))
+        check(lib().d2mi_distances(self._h, selected_phase, len(obj), ptr(obj, _lib._ip),
+                                   ptr(cent, _lib._ip), ptr(out)))
+        return out
+
+    def elkan_reset(self, selected_phase: int = -1):
+        check(lib().d2mi_elkan_reset(self._h, selected_phase))
+
+    def labeling(self, use_elkan: bool = False, selected_phase: int = -1) -> dict:
+        """d2_labeling (brute force) or d2_labeling_prep (Elkan) on the device."""
+        st = _lib.LabelStats()
+        check(lib().d2mi_labeling(self._h, selected_phase, int(use_elkan), C.byref(st)))
+        return dict(changes=st.changes, dist_count=st.dist_count, cost=st.cost, ms=st.ms)
+
+    def save_centroids(self, selected_phase: int = -1):
+        check(lib().d2mi_save_centroids(self._h, selected_phase))
+
+    def elkan_post(self, selected_phase: int = -1):
+        check(lib().d2mi_elkan_post(self._h, selected_phase))
+
+    def clustering(self, K: int, max_iter: int, opts: BADMMOptions = None, use_triangle=True,
+                   selected_phase: int = -1, precision: int = 32) -> dict:
+        """The d2_clustering outer loop (clustering.c:409-525) on the device,
+        from the resident centroids; labels are read back at the end."""
+        opts = opts or BADMMOptions()
+        st = _lib.ClusterStats()
+        check(lib().d2mi_clustering(self._h, K, max_iter, selected_phase, int(use_triangle),
+                                    opts.maxIters, opts.rhoCoeff, opts.updatePerLoops, precision,
+                                    C.byref(st)))
+        it = st.iterations
+        return dict(iterations=it, changes=list(st.changes[:it]),
+                    dist_count=list(st.dist_count[:it]), final_cost=st.final_cost,
+                    ms_labeling=st.ms_labeling, ms_badmm=st.ms_badmm)

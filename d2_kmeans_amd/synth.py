@@ -102,7 +102,7 @@ def grid_strokes(n=200_000, grid=16, n_templates=10, m_lo=20, m_hi=80, jitter=0.
     w = w / np.repeat(np.add.reduceat(w, starts), p_str)
     ph = Phase(metric=D2_EUCLIDEAN_L2, dim=2, p_str=p_str, p_w=w, p_supp=pts.reshape(-1), str=40)
     return D2Data([ph]), dict(kind="grid_strokes", n=n, grid=grid, n_templates=n_templates,
-                              m=(m_lo, m_hi), jitter=jitter, seed=seed)
+                              m=(m_lo, m_hi), jitter=jitter, seed=seed, classes=cls.astype(np.int32))
 
 
 def _zipf_trunc(rng, a, lo, hi, size):
@@ -150,3 +150,28 @@ def serving_requests(n=1000, d=64, n_comp=64, m_lo=5, m_hi=500, mu=np.log(50.0),
     ph = Phase(metric=D2_EUCLIDEAN_L2, dim=d, p_str=p_str, p_w=w, p_supp=pts.reshape(-1), str=48)
     return D2Data([ph]), dict(kind="serving_requests", n=n, d=d, m=(m_lo, m_hi), mu=float(mu),
                               sigma=sigma, seed=seed)
+
+
+def seed_centroids(phase: Phase, K: int, s: int, labels=None, seed=SEED_BASE):
+    """Starting centroids for benchmarks: for each cluster k, the first
+    object (in a seeded order, restricted to label k when labels are given)
+    with at least s supports, cut to its s heaviest supports, weights
+    renormalised.  (The clustering driver's own initialisation restates
+    d2_centroid_rands; this helper only needs a valid, seeded start.)"""
+    from .data import Centroids
+    rng = np.random.default_rng(seed)
+    order = rng.permutation(phase.size)
+    c = Centroids.zeros(K, s, phase.dim, with_supp=phase.dim > 0)
+    for k in range(K):
+        cand = order if labels is None else order[np.asarray(labels)[order] == k]
+        cand = cand[phase.p_str[cand] >= s]
+        if labels is None:
+            cand = cand[k:]
+        if len(cand) == 0:
+            raise ValueError(f"no object with >= {s} supports for cluster {k}")
+        w, sup = phase.object(int(cand[0]))
+        top = np.sort(np.argsort(-w, kind="stable")[:s])
+        c.p_w[k * s:(k + 1) * s] = w[top] / w[top].sum()
+        if c.p_supp is not None:
+            c.p_supp[k * s * phase.dim:(k + 1) * s * phase.dim] = sup[top].reshape(-1)
+    return c

@@ -105,6 +105,68 @@ int d2mi_set_plan(d2mi_ctx *ctx, int phase, int precision, const double *Z);
 int d2mi_pdist2(d2mi_ctx *ctx, int d, size_t n, size_t m, const double *A, const double *B,
                 double *C);
 
+/* Exact transport LP values of `nbatch` independent problems of one shape,
+ * the LP the reference's d2_match_by_distmat poses to Mosek
+ * (solver_mosek.cc:61-143, include/d2/solver.h:14-15):
+ *   min sum C[b][i + j*n] x_ij, x >= 0, row sums = wX[b][i] (n rows),
+ *   column sums = wY[b][j] for j < m-1 (the last column constraint dropped).
+ * C: nbatch*n*m, wX: nbatch*n, wY: nbatch*m host arrays; out[b] = optimum
+ * (FP64, solved on the device); pivots (may be NULL) = simplex pivots. */
+int d2mi_transport_batch(d2mi_ctx *ctx, int n, int m, int nbatch, const double *C,
+                         const double *wX, const double *wY, double *out, int *pivots);
+
+/* ---------------------------------------------------------- assignment */
+
+typedef struct {
+  unsigned long long changes;    /* objects whose label changed */
+  unsigned long long dist_count; /* exact distances computed (clustering.c:237) */
+  double cost;                   /* sum of min d^2 (brute force only; NaN with Elkan) */
+  double ms;                     /* device time */
+} d2mi_label_stats;
+
+/* Batched d2_compute_distance (clustering.c:33-128) between resident objects
+ * and resident centroids: out[t] = sqrt(sum over the selected phases of the
+ * exact transport-LP value between object obj[t] and centroid cent[t]).
+ * selected_phase < 0 means all phases. */
+int d2mi_distances(d2mi_ctx *ctx, int selected_phase, size_t npairs, const int *obj,
+                   const int *cent, double *out);
+
+/* Resets the resident Elkan bounds: l = 0, u = +max, r = 1
+ * (d2_allocate_work with use_triangle, clustering_util.c:218-228). */
+int d2mi_elkan_reset(d2mi_ctx *ctx, int selected_phase);
+
+/* Relabels every object against the resident centroids: use_elkan = 0 is
+ * d2_labeling (clustering.c:324-376, brute force, first minimum wins, cost
+ * = sum of min d^2); use_elkan = 1 is d2_labeling_prep (clustering.c:141-242)
+ * with the bounds resident on the device.  Labels and label-switch flags are
+ * updated on the device; stats may be NULL. */
+int d2mi_labeling(d2mi_ctx *ctx, int selected_phase, int use_elkan, d2mi_label_stats *stats);
+
+/* Keeps a copy of the current centroids (d2_copy before the update,
+ * clustering.c:484) for d2mi_elkan_post. */
+int d2mi_save_centroids(d2mi_ctx *ctx, int selected_phase);
+
+/* d2_labeling_post (clustering.c:293-318): centroid drifts between the saved
+ * and the current centroids relax the resident bounds. */
+int d2mi_elkan_post(d2mi_ctx *ctx, int selected_phase);
+
+typedef struct {
+  int iterations;                /* outer iterations run */
+  long long changes[256];        /* label changes per outer iteration */
+  long long dist_count[256];     /* distances computed per outer iteration */
+  double final_cost;             /* mean of min d^2 of the closing labeling */
+  double ms_labeling, ms_badmm;  /* device time per stage */
+} d2mi_cluster_stats;
+
+/* The d2_clustering outer loop (clustering.c:409-525) on resident data and
+ * centroids: labels start at -1; each iteration relabels (Elkan when
+ * use_triangle), stops when fewer than 0.001*N labels change, and runs one
+ * B-ADMM update per selected phase; with use_triangle a closing brute-force
+ * labeling gives the final cost.  B-ADMM options as in d2mi_badmm. */
+int d2mi_clustering(d2mi_ctx *ctx, int K, int max_iter, int selected_phase, int use_triangle,
+                    int maxIters, double rhoCoeff, int updatePerLoops, int precision,
+                    d2mi_cluster_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif
