@@ -87,6 +87,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-gate", action="store_true")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="also run one FP32->FP64 plan copy kernel (reads 4 B, writes 8 B per "
+                         "plan element) to scale rocprof byte counters")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -168,6 +171,11 @@ def main():
         plain = {"value": ph.size * passes / pw, "ms_per_step": 1000 * pw,
                  "plan_kernel_ms_avg": stp["ms_plan"] / stp["n_plan_launches"]}
         log(f"[bench] plain form: {plain}")
+
+    if args.copy_calib:
+        eng.get_plan(0, 32, "Z")
+        log(f"[bench] copy calibration: k_f32_to_f64 over {int(elems)} elements "
+            f"({4 * elems / 1e9:.3f} GB read, {8 * elems / 1e9:.3f} GB written)")
 
     cpu = None
     if not args.no_cpu_baseline:

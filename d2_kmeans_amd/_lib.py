@@ -10,7 +10,7 @@ import os
 import numpy as np
 
 PKG = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(PKG, "libd2mi.so")
+LIB_PATH = os.environ.get("D2MI_LIB") or os.path.join(PKG, "libd2mi.so")
 
 D2MI_ENGINE = 0
 D2MI_PLAIN = 1

@@ -23,28 +23,30 @@ FLAGS = [
 ]
 
 
-def build(verbose: bool = False, jobs: int = 4) -> str:
-    objdir = os.path.join(PKG, "_obj")
+def build(verbose: bool = False, jobs: int = 4, out: str = LIB, extra=()) -> str:
+    """Compiles every source and links ``out``; ``extra`` hipcc flags (for
+    example -D defines) apply to every translation unit."""
+    objdir = os.path.join(PKG, "_obj" if out == LIB else "_obj_" + os.path.basename(out))
     os.makedirs(objdir, exist_ok=True)
     procs = []
     objs = []
     for src in SOURCES:
         obj = os.path.join(objdir, src.replace(".hip", ".o"))
         objs.append(obj)
-        cmd = [HIPCC, *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+        cmd = [HIPCC, *FLAGS, *extra, "-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
         if len(procs) >= jobs:
             _wait(procs)
     _wait(procs)
-    tmp = LIB + ".tmp"
+    tmp = out + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.check_call(cmd)
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, out)
+    return out
 
 
 def _wait(procs):

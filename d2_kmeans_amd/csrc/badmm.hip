@@ -20,6 +20,7 @@
 // ranges do not interleave.  Every sum has a fixed order: results are
 // bit-reproducible run to run.
 #include <cmath>
+#include <cstdlib>
 #include <type_traits>
 
 #include "d2mi_internal.h"
@@ -60,20 +61,20 @@ __host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~size_t(
 
 // LDS layout of the plan kernel, shared by host (sizing) and device.
 struct PlanLds {
-  size_t A, B, X, f, w, xs, cs, cw, g, rr, ids, acc, red, total;
+  size_t A, B, X, w, xs, cs, cw, g, rr, rp, ids, acc, red, total;
   __host__ __device__ PlanLds(int tsize, int sp, int m_max, int s, int d, int keepx, int R) {
     const size_t EP = (size_t)sp * m_max;
     size_t o = 0;
     A = o; o = align16(o + EP * tsize);
     B = o; o = align16(o + EP * tsize);
     X = o; o = align16(o + (keepx ? EP : 0) * tsize);
-    f = o; o = align16(o + (size_t)m_max * tsize);
     w = o; o = align16(o + (size_t)m_max * tsize);
     xs = o; o = align16(o + (size_t)m_max * (d > 0 ? d : 1) * tsize);
     cs = o; o = align16(o + (size_t)s * (d > 0 ? d : 1) * tsize);
     cw = o; o = align16(o + (size_t)s * tsize);
     g = o; o = align16(o + (size_t)s * tsize);
     rr = o; o = align16(o + (size_t)s * tsize);
+    rp = o; o = align16(o + (size_t)4 * s * tsize);
     ids = o; o = align16(o + (size_t)m_max * sizeof(int));
     acc = o; o = align16(o + (size_t)R * sizeof(double));
     red = o; o = align16(o + 8 * sizeof(double));
@@ -102,15 +103,24 @@ struct PlanArgs {
   double *__restrict__ resid_part;  // [G * 3]
 };
 
-// ground cost of element (k, j) before the division by rho
-template <typename T, int CM>
+// ground cost of element (k, j) before the division by rho; DIM > 0 fixes
+// the Euclidean dimension at compile time (straight-line code)
+template <typename T, int CM, int DIM = 0>
 __device__ __forceinline__ T cost_raw(int k, int j, int s, int d, int V, const T *cs,
                                       const T *xs, const int *ids, const T *__restrict__ dist) {
   if constexpr (CM == COST_EUCLID) {
     T acc = 0;
-    for (int dd = 0; dd < d; ++dd) {
-      const T u = cs[k * d + dd] - xs[j * d + dd];
-      acc += u * u;
+    if constexpr (DIM > 0) {
+#pragma unroll
+      for (int dd = 0; dd < DIM; ++dd) {
+        const T u = cs[k * DIM + dd] - xs[j * DIM + dd];
+        acc += u * u;
+      }
+    } else {
+      for (int dd = 0; dd < d; ++dd) {
+        const T u = cs[k * d + dd] - xs[j * d + dd];
+        acc += u * u;
+      }
     }
     return acc;
   } else if constexpr (CM == COST_HIST) {
@@ -126,16 +136,37 @@ __device__ __forceinline__ T scale_cost(T c, double rho, T inv_rho) {
   else return c * inv_rho;
 }
 
-template <typename T, int CM>
-__global__ __launch_bounds__(NT) void k_badmm_plan(PlanArgs<T> a) {
+constexpr int PRE_W = 2;  // register slots for weights / bin ids: m_max <= 512
+constexpr int PRE_X = 4;  // register slots for supports: m_max * d <= 1024
+
+#ifndef D2MI_PLAN_WAVES_PER_EU
+#define D2MI_PLAN_WAVES_PER_EU 1
+#endif
+
+// Plan update of every object in this workgroup's chunk of the label-sorted
+// order.  Per object: its Z, Y (RMAX register slots of NT elements; RMAX == 0
+// loads inside the first pass), weights and supports are requested at the
+// top, before the workgroup barrier, so their HBM round trips overlap; then
+//   pass 1 (element per thread): t = Z exp(-(C + Y)) + 1e-9 into LDS;
+//   pass 2 (four threads per column): column sums by quad shuffles, then the
+//          same threads form x = t / colsum * w and z' = x exp(Y) + 1e-9;
+//   pass 3 (four waves): partial row sums, combined by wave 0 into the
+//          normalisers and the per-label sums;
+//   pass 4 (element per thread): z = z' / rowsum * c_w, Y += x - z, stored.
+template <typename T, int CM, int RMAX, int DIM>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(D2MI_PLAN_WAVES_PER_EU)))
+void k_badmm_plan(PlanArgs<T> a, const int *__restrict__ g_perm, const int *__restrict__ g_str,
+                  const long long *__restrict__ g_cum, const int *__restrict__ g_label) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr bool F64 = std::is_same<T, double>::value;
-  const int s = a.s, d = a.d, sp = a.sp, tid = threadIdx.x;
+  constexpr int RS = RMAX > 0 ? RMAX : 1;
+  const int s = a.s, d = DIM > 0 ? DIM : a.d, sp = a.sp, tid = threadIdx.x;
+  const int lane = tid & 63, wv = tid >> 6;
   const PlanLds L(sizeof(T), sp, a.m_max, s, d, a.keepx, a.R);
   T *A = (T *)(smem + L.A), *B = (T *)(smem + L.B), *Xs = (T *)(smem + L.X);
-  T *f = (T *)(smem + L.f), *wl = (T *)(smem + L.w), *xs = (T *)(smem + L.xs);
+  T *wl = (T *)(smem + L.w), *xs = (T *)(smem + L.xs);
   T *cs = (T *)(smem + L.cs), *cwl = (T *)(smem + L.cw), *g = (T *)(smem + L.g);
-  T *rr = (T *)(smem + L.rr);
+  T *rr = (T *)(smem + L.rr), *rp = (T *)(smem + L.rp);
   int *ids = (int *)(smem + L.ids);
   double *acc = (double *)(smem + L.acc), *red = (double *)(smem + L.red);
 
@@ -144,17 +175,48 @@ __global__ __launch_bounds__(NT) void k_badmm_plan(PlanArgs<T> a) {
   const double rho = a.scal[0];
   const T inv_rho = (T)a.scal[1];
   const int dk = NT % s, dj = NT / s, k0 = tid % s, j0 = tid / s;
+  const bool small_s = s <= 64;  // row sums split over the 4 waves
   double r_obj = 0.0, r_pr = 0.0, r_dr = 0.0;
   for (int q = tid; q < a.R; q += NT) acc[q] = 0.0;
   int cur = -1;
 
   for (int p = p0; p < p1; ++p) {
-    const int o = a.perm[p];
-    const int l = a.label[o];
-    const int m = a.p_str[o];
-    const long long off = a.cum[o];
+    const int o = g_perm[p];
+    const int l = g_label[o];
+    const int m = g_str[o];
+    const long long off = g_cum[o];
+    const int E = s * m;
+    const size_t base = (size_t)s * off;
+    // requests first: plan, weights, supports of this object
+    T zr[RS], yr[RS], wpre[PRE_W], xpre[PRE_X];
+    int ipre[PRE_W];
+    if constexpr (RMAX > 0) {
+#pragma unroll
+      for (int r = 0; r < RMAX; ++r) {
+        const int e = tid + r * NT;
+        if (e < E) {
+          zr[r] = a.Z[base + e];
+          yr[r] = a.Y[base + e];
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < PRE_W; ++r) {
+      const int q = tid + r * NT;
+      if (q < m) {
+        wpre[r] = a.w[off + q];
+        if constexpr (CM == COST_SPARSE) ipre[r] = a.sym[off + q];
+      }
+    }
+    if constexpr (CM == COST_EUCLID) {
+#pragma unroll
+      for (int r = 0; r < PRE_X; ++r) {
+        const int q = tid + r * NT;
+        if (q < m * d) xpre[r] = a.supp[(size_t)off * d + q];
+      }
+    }
     if (l != cur) {
-      __syncthreads();  // acc of the previous label is complete
+      __syncthreads();  // every wave is done with the previous label
       if (cur >= 0)
         for (int q = tid; q < a.R; q += NT) {
           a.rec[(size_t)(gi + cur) * a.R + q] = acc[q];
@@ -165,61 +227,91 @@ __global__ __launch_bounds__(NT) void k_badmm_plan(PlanArgs<T> a) {
       if constexpr (CM == COST_EUCLID)
         for (int q = tid; q < s * d; q += NT) cs[q] = a.csupp[(size_t)l * s * d + q];
     }
-    for (int q = tid; q < m; q += NT) wl[q] = a.w[off + q];
-    if constexpr (CM == COST_EUCLID)
-      for (int q = tid; q < m * d; q += NT) xs[q] = a.supp[(size_t)off * d + q];
-    if constexpr (CM == COST_SPARSE)
-      for (int q = tid; q < m; q += NT) ids[q] = a.sym[off + q];
-    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < PRE_W; ++r) {
+      const int q = tid + r * NT;
+      if (q < m) {
+        wl[q] = wpre[r];
+        if constexpr (CM == COST_SPARSE) ids[q] = ipre[r];
+      }
+    }
+    if constexpr (CM == COST_EUCLID) {
+#pragma unroll
+      for (int r = 0; r < PRE_X; ++r) {
+        const int q = tid + r * NT;
+        if (q < m * d) xs[q] = xpre[r];
+      }
+    }
+    __syncthreads();  // also: every wave has finished the previous object's last pass
 
-    const int E = s * m;
-    const size_t base = (size_t)s * off;
-    // step 1a: t = Z exp(-(C + Y)) + 1e-9 (:155-157)
+    // pass 1 -- step 1a: t = Z exp(-(C + Y)) + 1e-9 (:155-157)
     {
       int k = k0, j = j0;
-      for (int e = tid; e < E; e += NT) {
-        const T c = scale_cost<T>(cost_raw<T, CM>(k, j, s, d, a.V, cs, xs, ids, a.dist), rho,
-                                  inv_rho);
-        const T z = a.Z[base + e], y = a.Y[base + e];
-        const int li = k + j * sp;
-        A[li] = z * dexp<T>(-(c + y)) + (T)1e-9;
-        B[li] = y;
-        k += dk;
-        j += dj;
-        if (k >= s) { k -= s; ++j; }
+      if constexpr (RMAX > 0) {
+#pragma unroll
+        for (int r = 0; r < RMAX; ++r) {
+          const int e = tid + r * NT;
+          if (e < E) {
+            const T c = scale_cost<T>(cost_raw<T, CM, DIM>(k, j, s, d, a.V, cs, xs, ids, a.dist),
+                                      rho, inv_rho);
+            const int li = k + j * sp;
+            A[li] = zr[r] * dexp<T>(-(c + yr[r])) + (T)1e-9;
+            B[li] = yr[r];
+          }
+          k += dk;
+          j += dj;
+          if (k >= s) { k -= s; ++j; }
+        }
+      } else {
+        for (int e = tid; e < E; e += NT) {
+          const T c = scale_cost<T>(cost_raw<T, CM, DIM>(k, j, s, d, a.V, cs, xs, ids, a.dist),
+                                    rho, inv_rho);
+          const T z = a.Z[base + e], y = a.Y[base + e];
+          const int li = k + j * sp;
+          A[li] = z * dexp<T>(-(c + y)) + (T)1e-9;
+          B[li] = y;
+          k += dk;
+          j += dj;
+          if (k >= s) { k -= s; ++j; }
+        }
       }
     }
     __syncthreads();
-    // step 1b: column sums (_dcnorm, :158)
-    for (int j = tid; j < m; j += NT) {
-      T acc_c = 0;
-      const T *colp = A + j * sp;
-      for (int k = 0; k < s; ++k) acc_c += colp[k];
-      f[j] = F64 ? acc_c : wl[j] / acc_c;
-    }
-    __syncthreads();
-    // step 1c + 2a: x = t / colsum * w (:158-159); z' = x exp(Y) + 1e-9 (:166)
-    {
-      int k = k0, j = j0;
-      for (int e = tid; e < E; e += NT) {
+    // pass 2 -- step 1b/1c: column sums (_dcnorm, :158) by quads, x = t / colsum
+    // * w (:159); step 2a: z' = x exp(Y) + 1e-9 (:166)
+    for (int t = tid; t < 4 * m; t += NT) {
+      const int j = t >> 2, q = t & 3;
+      T* colA = A + j * sp;
+      T part = 0;
+      for (int k = q; k < s; k += 4) part += colA[k];
+      part += __shfl_xor(part, 1, 64);
+      part += __shfl_xor(part, 2, 64);
+      const T wj = wl[j];
+      const T scale = F64 ? (T)0 : wj / part;
+      for (int k = q; k < s; k += 4) {
         const int li = k + j * sp;
-        const T x = F64 ? (A[li] / f[j]) * wl[j] : A[li] * f[j];
+        const T x = F64 ? (colA[k] / part) * wj : colA[k] * scale;
         const T y = B[li];
-        A[li] = x * dexp<T>(y) + (T)1e-9;
+        colA[k] = x * dexp<T>(y) + (T)1e-9;
         if (a.keepx) Xs[li] = x;
         else B[li] = y + x;
-        k += dk;
-        j += dj;
-        if (k >= s) { k -= s; ++j; }
       }
     }
     __syncthreads();
-    // step 2b: row sums (_drnorm, :169); step 5 partials on update iterations
-    for (int k = tid; k < s; k += NT) {
-      T r = 0;
-      for (int j = 0; j < m; ++j) r += A[k + j * sp];
-      rr[k] = r;
-      g[k] = F64 ? r : cwl[k] / r;
+    // pass 3 -- step 2b: row sums (_drnorm, :169) split over the four waves;
+    // step 5 partials on support-update iterations (:206-211)
+    if (small_s) {
+      if (lane < s) {
+        T part = 0;
+        for (int j = wv; j < m; j += 4) part += A[lane + j * sp];
+        rp[wv * s + lane] = part;
+      }
+    } else {
+      for (int k = tid; k < s; k += NT) {
+        T r = 0;
+        for (int j = 0; j < m; ++j) r += A[k + j * sp];
+        rr[k] = r;
+      }
     }
     if (a.upd) {
       for (int k = tid; k < s; k += NT) {
@@ -228,8 +320,8 @@ __global__ __launch_bounds__(NT) void k_badmm_plan(PlanArgs<T> a) {
         double rx = 0.0;
         for (int j = 0; j < m; ++j) {
           const double x = (double)Xs[k + j * sp];
-          rx += x;  // row sums of X (:211)
-          for (int dd = 0; dd < d; ++dd) P[dd] += x * (double)xs[j * d + dd];  // supp X^T (:206)
+          rx += x;
+          for (int dd = 0; dd < d; ++dd) P[dd] += x * (double)xs[j * d + dd];
         }
         for (int dd = 0; dd < d; ++dd) acc[s + k * d + dd] += P[dd];
         acc[s + s * d + k] += rx;
@@ -237,19 +329,36 @@ __global__ __launch_bounds__(NT) void k_badmm_plan(PlanArgs<T> a) {
     }
     __syncthreads();
     // step 4: normalised row sums summed per label (:180-185)
-    double rsum = 0.0;
-    for (int k = tid; k < s; k += NT) rsum += (double)rr[k];
-    rsum = block_sum(rsum, red);
-    for (int k = tid; k < s; k += NT) acc[k] += (double)rr[k] / rsum;
-    // step 2c + 3: z = z' / rowsum * c_w (:169-170); Y += X - Z (:175)
+    if (small_s) {
+      if (wv == 0) {
+        T r = 0;
+        if (lane < s) {
+          r = rp[lane] + rp[s + lane] + rp[2 * s + lane] + rp[3 * s + lane];
+          rr[lane] = r;
+          g[lane] = F64 ? r : cwl[lane] / r;
+        }
+        double rs = wave_sum(lane < s ? (double)r : 0.0);
+        if (lane < s) acc[lane] += (double)r / rs;
+      }
+    } else {
+      double rsum = 0.0;
+      for (int k = tid; k < s; k += NT) {
+        rsum += (double)rr[k];
+        g[k] = F64 ? rr[k] : cwl[k] / rr[k];
+      }
+      rsum = block_sum(rsum, red);
+      for (int k = tid; k < s; k += NT) acc[k] += (double)rr[k] / rsum;
+    }
+    __syncthreads();
+    // pass 4 -- step 2c + 3: z = z' / rowsum * c_w (:169-170); Y += X - Z (:175)
     {
       int k = k0, j = j0;
       for (int e = tid; e < E; e += NT) {
         const int li = k + j * sp;
         const T z = F64 ? (A[li] / g[k]) * cwl[k] : A[li] * g[k];
         if (a.resid) {  // step 6 (:300-305), and the exit objective
-          const T c = scale_cost<T>(cost_raw<T, CM>(k, j, s, d, a.V, cs, xs, ids, a.dist), rho,
-                                    inv_rho);
+          const T c = scale_cost<T>(cost_raw<T, CM, DIM>(k, j, s, d, a.V, cs, xs, ids, a.dist),
+                                    rho, inv_rho);
           const T x = Xs[li];
           const T z0 = a.Z[base + e];
           r_obj += (double)c * (double)x;
@@ -263,7 +372,7 @@ __global__ __launch_bounds__(NT) void k_badmm_plan(PlanArgs<T> a) {
         if (k >= s) { k -= s; ++j; }
       }
     }
-    __syncthreads();
+    if (a.resid) __syncthreads();  // pass 4 read xs / cs: keep them until all waves are done
   }
   __syncthreads();
   if (cur >= 0)
@@ -642,9 +751,31 @@ int d2mi_ensure_plans(d2mi_ctx *ctx, PhaseDev *p, int precision) {
 
 namespace {
 
-struct Launch {
-  int G, chunk;
-};
+// resident workgroups per CU of the plan kernel at this LDS size (HIP's
+// occupancy calculator), so one launch is exactly one round of workgroups
+template <typename T, int CM, int DIM>
+int plan_occupancy(size_t lds) {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_badmm_plan<T, CM, 0, DIM>, NT, lds) !=
+          hipSuccess || n <= 0)
+    n = 1;
+  return n;
+}
+
+template <typename T, int CM, int DIM>
+void launch_plan(int rmax, int G, size_t lds, hipStream_t S, const PlanArgs<T> &pa) {
+#ifndef D2MI_PLAN_EARLY_LOADS
+  // Measured on MI355X: holding a whole object's Z/Y in registers costs more
+  // occupancy than it saves in latency, so by default loads stay in pass 1.
+  rmax = 0;
+#endif
+  switch (rmax) {
+    case 8: k_badmm_plan<T, CM, 8, DIM><<<G, NT, lds, S>>>(pa, pa.perm, pa.p_str, pa.cum, pa.label); break;
+    case 16: k_badmm_plan<T, CM, 16, DIM><<<G, NT, lds, S>>>(pa, pa.perm, pa.p_str, pa.cum, pa.label); break;
+    case 32: k_badmm_plan<T, CM, 32, DIM><<<G, NT, lds, S>>>(pa, pa.perm, pa.p_str, pa.cum, pa.label); break;
+    default: k_badmm_plan<T, CM, 0, DIM><<<G, NT, lds, S>>>(pa, pa.perm, pa.p_str, pa.cum, pa.label); break;
+  }
+}
 
 template <typename T, int CM>
 int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int updatePerLoops,
@@ -726,14 +857,35 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
   double ms_plan = 0.0;
 
   if (variant == D2MI_BADMM_ENGINE) {
-    // ---- launch geometry: contiguous chunks of the label-sorted order
-    const int Gt = std::max(1, ctx->num_cu * 8);
-    const int chunk = std::max(1, (N + Gt - 1) / Gt);
-    const int G = (N + chunk - 1) / chunk;
     const int sp = (s % 2 == 0) ? s + 1 : s;  // odd LDS column stride: conflict-free sums
     const int Rmax = euclid_upd ? s * (d + 2) : s;
-    D2MI_TRY(ctx->rec.ensure(sizeof(double) * (size_t)(G + K) * Rmax));
-    D2MI_TRY(ctx->part.ensure(sizeof(double) * (size_t)std::max(setup_blocks, 3 * G)));
+    // launch geometry: contiguous chunks of the label-sorted order, one chunk
+    // per resident workgroup (D2MI_PLAN_WG_PER_CU overrides, for tuning)
+    static const int wg_env = [] {
+      const char *e = getenv("D2MI_PLAN_WG_PER_CU");
+      return e ? atoi(e) : 0;
+    }();
+    auto geometry = [&](size_t lds, int *G_out, int *chunk_out) {
+      int per_cu;
+      if (wg_env > 0) per_cu = wg_env;
+      else if (CM == COST_EUCLID && d == 2) per_cu = plan_occupancy<T, CM, 2>(lds);
+      else if (CM == COST_EUCLID && d == 3) per_cu = plan_occupancy<T, CM, 3>(lds);
+      else per_cu = plan_occupancy<T, CM, 0>(lds);
+      const int Gt = std::max(1, ctx->num_cu * per_cu);
+      *chunk_out = std::max(1, (N + Gt - 1) / Gt);
+      *G_out = (N + *chunk_out - 1) / *chunk_out;
+    };
+    int Gmax = 0, chunk_tmp = 0;
+    geometry(PlanLds(sizeof(T), sp, m_max, s, d, 0, s).total, &Gmax, &chunk_tmp);
+    D2MI_TRY(ctx->rec.ensure(sizeof(double) * (size_t)(Gmax + K) * Rmax));
+    D2MI_TRY(ctx->part.ensure(sizeof(double) * (size_t)std::max(setup_blocks, 3 * Gmax)));
+    if (m_max > NT * PRE_W || (size_t)m_max * std::max(d, 1) > (size_t)NT * PRE_X) {
+      d2mi_set_error("objects of up to %d supports (d=%d) exceed this build's plan kernel "
+                     "(m <= %d, m*d <= %d)", m_max, d, NT * PRE_W, NT * PRE_X);
+      return D2MI_EUNSUPPORTED;
+    }
+    const int slots = (s * m_max + NT - 1) / NT;
+    const int rmax = slots <= 8 ? 8 : slots <= 16 ? 16 : slots <= 32 ? 32 : 0;
     const size_t lds_full = PlanLds(sizeof(T), sp, m_max, s, d, 1, Rmax).total;
     if (lds_full > ctx->lds_max) {
       d2mi_set_error("object plan s*m_max = %d x %d needs %zu B of LDS (limit %zu)", s, m_max,
@@ -747,6 +899,13 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
       const int resid = is_resid(iter);
       const int keepx = F64 || upd || resid;
       const int R = upd ? s * (d + 2) : s;
+      const size_t lds = PlanLds(sizeof(T), sp, m_max, s, d, keepx, R).total;
+      int G, chunk;
+      geometry(lds, &G, &chunk);
+      if (G > Gmax) {  // records are sized for Gmax workgroups
+        chunk = std::max(1, (N + Gmax - 1) / Gmax);
+        G = (N + chunk - 1) / chunk;
+      }
       PlanArgs<T> pa{};
       pa.N = N; pa.s = s; pa.d = d; pa.K = K; pa.sp = sp; pa.m_max = m_max; pa.chunk = chunk;
       pa.R = R; pa.V = p->V; pa.upd = upd; pa.resid = resid; pa.keepx = keepx;
@@ -754,9 +913,14 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
       pa.label = ctx->label.as<int>(); pa.w = w; pa.supp = supp; pa.sym = sym; pa.dist = dist;
       pa.cw = cwT; pa.csupp = csT; pa.scal = scal; pa.Z = Z; pa.Y = Y;
       pa.rec = ctx->rec.as<double>(); pa.resid_part = ctx->part.as<double>();
-      const size_t lds = PlanLds(sizeof(T), sp, m_max, s, d, keepx, R).total;
       D2MI_HIP(hipEventRecord(evs[2 * iter], S));
-      k_badmm_plan<T, CM><<<G, NT, lds, S>>>(pa);
+      if constexpr (CM == COST_EUCLID) {
+        if (d == 2) launch_plan<T, CM, 2>(rmax, G, lds, S, pa);
+        else if (d == 3) launch_plan<T, CM, 3>(rmax, G, lds, S, pa);
+        else launch_plan<T, CM, 0>(rmax, G, lds, S, pa);
+      } else {
+        launch_plan<T, CM, 0>(rmax, G, lds, S, pa);
+      }
       D2MI_HIP(hipEventRecord(evs[2 * iter + 1], S));
       FinArgs<T> fa{};
       fa.K = K; fa.s = s; fa.d = d; fa.chunk = chunk; fa.R = R; fa.G = G; fa.upd = upd;

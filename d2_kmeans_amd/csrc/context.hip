@@ -158,14 +158,17 @@ extern "C" int d2mi_set_phase(d2mi_ctx *ctx, int phase, int metric, int dim, siz
     d2mi_set_error("d2mi_set_phase: metric %d not supported", metric);
     return D2MI_EUNSUPPORTED;
   }
-  if (ctx->N && size != ctx->N) {
-    bool other = false;
-    for (size_t k = 0; k < ctx->ph.size(); ++k)
-      if ((int)k != phase && ctx->ph[k].set) other = true;
-    if (other) {
-      d2mi_set_error("phase %d has %zu objects, other phases %zu", phase, size, ctx->N);
-      return D2MI_EINVAL;
-    }
+  if (phase == 0) {
+    // phase 0 starts a new data set: drop the other phases and per-object state
+    for (size_t k = 1; k < ctx->ph.size(); ++k) release_phase(ctx->ph[k]);
+    if (!ctx->ph.empty()) ctx->ph.resize(1);
+    ctx->label.release();
+    ctx->lsw.release();
+    ctx->elkan_K = 0;
+    ctx->N = 0;
+  } else if (ctx->N && size != ctx->N) {
+    d2mi_set_error("phase %d has %zu objects, phase 0 has %zu", phase, size, ctx->N);
+    return D2MI_EINVAL;
   }
   // validate the CSR layout on the host: the kernels trust it
   size_t acc = 0;

@@ -54,7 +54,9 @@ int d2mi_synchronize(d2mi_ctx *ctx);
 /* Uploads one phase of the data set (the `sph` filled by d2_read,
  * clustering_io.c:14-202).  Arrays are copied; the caller keeps ownership.
  * Unused arrays may be NULL (p_supp for symbolic metrics, etc.).  Replaces
- * any earlier upload of that phase and discards its B-ADMM state. */
+ * any earlier upload of that phase and discards its B-ADMM state.  Uploading
+ * phase 0 starts a new data set: other phases, labels and Elkan bounds are
+ * dropped; phases > 0 must then hold the same number of objects. */
 int d2mi_set_phase(d2mi_ctx *ctx, int phase, int metric, int dim, size_t size, size_t col,
                    const int *p_str, const size_t *p_str_cum, const double *p_w,
                    const double *p_supp, const int *p_supp_sym, int vocab_size,
