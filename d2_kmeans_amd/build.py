@@ -10,7 +10,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libd2mi.so")
-SOURCES = ["context.hip", "badmm.hip", "transport.hip", "labeling.hip"]
+SOURCES = ["context.hip", "badmm.hip", "transport.hip", "labeling.hip", "compat.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("D2MI_ARCH", "gfx950")
 
@@ -46,7 +46,24 @@ def build(verbose: bool = False, jobs: int = 4, out: str = LIB, extra=()) -> str
         print(" ".join(cmd), flush=True)
     subprocess.check_call(cmd)
     os.replace(tmp, out)
+    if out == LIB:
+        build_cli(verbose)
     return out
+
+
+CLI = os.path.join(PKG, "d2mi")
+
+
+def build_cli(verbose: bool = False) -> str:
+    """The d2 command line (reference flags) linked against libd2mi.so."""
+    cmd = ["g++", "-O2", "-std=c++17", f"-I{os.path.join(ROOT, 'include')}",
+           os.path.join(CSRC, "d2_cli.cpp"), f"-L{PKG}", "-ld2mi", "-Wl,-rpath,$ORIGIN",
+           "-o", CLI + ".tmp"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+    os.replace(CLI + ".tmp", CLI)
+    return CLI
 
 
 def _wait(procs):

@@ -15,7 +15,7 @@ def declared():
         if fn.endswith(".h"):
             text = open(os.path.join(ROOT, "include", fn)).read()
             text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-            names |= set(re.findall(r"\b(d2mi_\w+)\s*\(", text))
+            names |= set(re.findall(r"\b(d2mi_\w+|d2_\w+|_dpdist2)\s*\(", text))
     return names
 
 
@@ -31,4 +31,12 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_bindings_cover_the_header():
-    assert declared() <= set(_lib.SIGNATURES), declared() - set(_lib.SIGNATURES)
+    engine_api = {n for n in declared() if n.startswith("d2mi_")}
+    assert engine_api <= set(_lib.SIGNATURES), engine_api - set(_lib.SIGNATURES)
+
+
+def test_reference_globals_exported():
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    for g in ("d2_alg_type", "p_badmm_options", "badmm_clu_options", "badmm_cen_options",
+              "world_rank", "nprocs"):
+        ctypes.c_int.in_dll(lib, g)
