@@ -52,6 +52,20 @@ class _Quiet:
         os.close(self.null)
 
 
+def profiled_traffic():
+    """HBM bytes per plan-kernel launch measured by rocprofv3 PMC counters
+    (FETCH_SIZE, WRITE_SIZE, scaled by a copy kernel) and committed under
+    profiles/; None when the summary is absent."""
+    fn = os.path.join(ROOT, "profiles", "k_badmm_plan_grid_strokes_pmc_traffic.txt")
+    try:
+        for line in open(fn):
+            if line.startswith("TRAFFIC_BYTES_PER_LAUNCH"):
+                return float(line.split()[1])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(data, labels, c0, opts, n_sample):
     """The reference's own B-ADMM (oracle/_ref, built from its sources,
     serial FP64, MKL sequential) on the first n_sample objects of the same
@@ -193,7 +207,7 @@ def main():
                                "(11 passes), rhoCoeff=2.0, supports every 10 passes",
                    "objects": ph.size, "plan_elements": int(elems), "K": K, "s": s},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": profiled_traffic(),
                      "kernel": "k_badmm_plan<float,EUCLID>",
                      "kernel_ms_avg": avg_plan_ms, "algorithmic_bytes_per_launch": bytes_per_launch},
         "gate_obj_rel_err": None if gate is None else gate["obj_rel_err"],

@@ -61,7 +61,7 @@ __host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~size_t(
 
 // LDS layout of the plan kernel, shared by host (sizing) and device.
 struct PlanLds {
-  size_t A, B, X, w, xs, cs, cw, g, rr, rp, ids, acc, red, total;
+  size_t A, B, X, w, xs, cs, cw, g, rr, rp, ids, acc, red, up, total;
   __host__ __device__ PlanLds(int tsize, int sp, int m_max, int s, int d, int keepx, int R) {
     const size_t EP = (size_t)sp * m_max;
     size_t o = 0;
@@ -78,6 +78,8 @@ struct PlanLds {
     ids = o; o = align16(o + (size_t)m_max * sizeof(int));
     acc = o; o = align16(o + (size_t)R * sizeof(double));
     red = o; o = align16(o + 8 * sizeof(double));
+    // support-update partials of the four waves (R > s only on those passes)
+    up = o; o = align16(o + (R > s ? (size_t)4 * s * ((d > 0 ? d : 1) + 1) : 0) * sizeof(double));
     total = o;
   }
 };
@@ -85,7 +87,8 @@ struct PlanLds {
 template <typename T>
 struct PlanArgs {
   int N, s, d, K, sp, m_max, chunk, R, V;
-  int upd, resid, keepx;
+  int upd, resid, keepx, first, force_init;
+  const char *lsw;
   const int *__restrict__ p_str;
   const long long *__restrict__ cum;
   const int *__restrict__ perm;
@@ -156,7 +159,8 @@ constexpr int PRE_X = 4;  // register slots for supports: m_max * d <= 1024
 template <typename T, int CM, int RMAX, int DIM>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(D2MI_PLAN_WAVES_PER_EU)))
 void k_badmm_plan(PlanArgs<T> a, const int *__restrict__ g_perm, const int *__restrict__ g_str,
-                  const long long *__restrict__ g_cum, const int *__restrict__ g_label) {
+                  const long long *__restrict__ g_cum, const int *__restrict__ g_label,
+                  const char *__restrict__ g_lsw) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr bool F64 = std::is_same<T, double>::value;
   constexpr int RS = RMAX > 0 ? RMAX : 1;
@@ -169,13 +173,14 @@ void k_badmm_plan(PlanArgs<T> a, const int *__restrict__ g_perm, const int *__re
   T *rr = (T *)(smem + L.rr), *rp = (T *)(smem + L.rp);
   int *ids = (int *)(smem + L.ids);
   double *acc = (double *)(smem + L.acc), *red = (double *)(smem + L.red);
+  double *up = (double *)(smem + L.up);
 
   const int gi = blockIdx.x;
   const int p0 = gi * a.chunk, p1 = min(a.N, p0 + a.chunk);
   const double rho = a.scal[0];
   const T inv_rho = (T)a.scal[1];
   const int dk = NT % s, dj = NT / s, k0 = tid % s, j0 = tid / s;
-  const bool small_s = s <= 64;  // row sums split over the 4 waves
+  const bool small_s = s <= 64;  // row and support sums split over the 4 waves
   double r_obj = 0.0, r_pr = 0.0, r_dr = 0.0;
   for (int q = tid; q < a.R; q += NT) acc[q] = 0.0;
   int cur = -1;
@@ -187,6 +192,9 @@ void k_badmm_plan(PlanArgs<T> a, const int *__restrict__ g_perm, const int *__re
     const long long off = g_cum[o];
     const int E = s * m;
     const size_t base = (size_t)s * off;
+    // first pass of an update: Y = 0 (:128) and, for objects whose label
+    // switched, Z = w_j * c_w[label][k] (:116-124), without touching HBM
+    const bool reinit = a.first && (a.force_init || g_lsw[o] == 1);
     // requests first: plan, weights, supports of this object
     T zr[RS], yr[RS], wpre[PRE_W], xpre[PRE_X];
     int ipre[PRE_W];
@@ -195,8 +203,8 @@ void k_badmm_plan(PlanArgs<T> a, const int *__restrict__ g_perm, const int *__re
       for (int r = 0; r < RMAX; ++r) {
         const int e = tid + r * NT;
         if (e < E) {
-          zr[r] = a.Z[base + e];
-          yr[r] = a.Y[base + e];
+          if (!reinit) zr[r] = a.Z[base + e];
+          yr[r] = a.first ? (T)0 : a.Y[base + e];
         }
       }
     }
@@ -255,7 +263,8 @@ void k_badmm_plan(PlanArgs<T> a, const int *__restrict__ g_perm, const int *__re
             const T c = scale_cost<T>(cost_raw<T, CM, DIM>(k, j, s, d, a.V, cs, xs, ids, a.dist),
                                       rho, inv_rho);
             const int li = k + j * sp;
-            A[li] = zr[r] * dexp<T>(-(c + yr[r])) + (T)1e-9;
+            const T z = reinit ? wl[j] * cwl[k] : zr[r];
+            A[li] = z * dexp<T>(-(c + yr[r])) + (T)1e-9;
             B[li] = yr[r];
           }
           k += dk;
@@ -266,7 +275,8 @@ void k_badmm_plan(PlanArgs<T> a, const int *__restrict__ g_perm, const int *__re
         for (int e = tid; e < E; e += NT) {
           const T c = scale_cost<T>(cost_raw<T, CM, DIM>(k, j, s, d, a.V, cs, xs, ids, a.dist),
                                     rho, inv_rho);
-          const T z = a.Z[base + e], y = a.Y[base + e];
+          const T z = reinit ? wl[j] * cwl[k] : a.Z[base + e];
+          const T y = a.first ? (T)0 : a.Y[base + e];
           const int li = k + j * sp;
           A[li] = z * dexp<T>(-(c + y)) + (T)1e-9;
           B[li] = y;
@@ -314,17 +324,33 @@ void k_badmm_plan(PlanArgs<T> a, const int *__restrict__ g_perm, const int *__re
       }
     }
     if (a.upd) {
-      for (int k = tid; k < s; k += NT) {
-        double P[MAXD_FUSED];
-        for (int dd = 0; dd < d; ++dd) P[dd] = 0.0;
-        double rx = 0.0;
-        for (int j = 0; j < m; ++j) {
-          const double x = (double)Xs[k + j * sp];
-          rx += x;
-          for (int dd = 0; dd < d; ++dd) P[dd] += x * (double)xs[j * d + dd];
+      if (small_s) {  // wave wv: columns j = wv, wv + 4, ...
+        if (lane < s) {
+          double P[MAXD_FUSED];
+          for (int dd = 0; dd < d; ++dd) P[dd] = 0.0;
+          double rx = 0.0;
+          for (int j = wv; j < m; j += 4) {
+            const double x = (double)Xs[lane + j * sp];
+            rx += x;  // row sums of X (:211)
+            for (int dd = 0; dd < d; ++dd) P[dd] += x * (double)xs[j * d + dd];  // supp X^T (:206)
+          }
+          double *u = up + (size_t)(wv * s + lane) * (d + 1);
+          for (int dd = 0; dd < d; ++dd) u[dd] = P[dd];
+          u[d] = rx;
         }
-        for (int dd = 0; dd < d; ++dd) acc[s + k * d + dd] += P[dd];
-        acc[s + s * d + k] += rx;
+      } else {
+        for (int k = tid; k < s; k += NT) {
+          double P[MAXD_FUSED];
+          for (int dd = 0; dd < d; ++dd) P[dd] = 0.0;
+          double rx = 0.0;
+          for (int j = 0; j < m; ++j) {
+            const double x = (double)Xs[k + j * sp];
+            rx += x;
+            for (int dd = 0; dd < d; ++dd) P[dd] += x * (double)xs[j * d + dd];
+          }
+          for (int dd = 0; dd < d; ++dd) acc[s + k * d + dd] += P[dd];
+          acc[s + s * d + k] += rx;
+        }
       }
     }
     __syncthreads();
@@ -339,6 +365,14 @@ void k_badmm_plan(PlanArgs<T> a, const int *__restrict__ g_perm, const int *__re
         }
         double rs = wave_sum(lane < s ? (double)r : 0.0);
         if (lane < s) acc[lane] += (double)r / rs;
+        if (a.upd && lane < s) {
+          for (int dd = 0; dd <= d; ++dd) {
+            double v = 0.0;
+            for (int w4 = 0; w4 < 4; ++w4) v += up[(size_t)(w4 * s + lane) * (d + 1) + dd];
+            if (dd < d) acc[s + lane * d + dd] += v;
+            else acc[s + s * d + lane] += v;
+          }
+        }
       }
     } else {
       double rsum = 0.0;
@@ -360,7 +394,7 @@ void k_badmm_plan(PlanArgs<T> a, const int *__restrict__ g_perm, const int *__re
           const T c = scale_cost<T>(cost_raw<T, CM, DIM>(k, j, s, d, a.V, cs, xs, ids, a.dist),
                                     rho, inv_rho);
           const T x = Xs[li];
-          const T z0 = a.Z[base + e];
+          const T z0 = reinit ? wl[j] * cwl[k] : a.Z[base + e];
           r_obj += (double)c * (double)x;
           r_pr += fabs((double)x - (double)z);
           r_dr += fabs((double)z0 - (double)z);
@@ -463,7 +497,7 @@ struct SetupArgs {
   const long long *__restrict__ cum;
   const int *__restrict__ label;
   const char *__restrict__ lsw;
-  int force_init;
+  int force_init, write_state;
   const T *__restrict__ w;
   const T *__restrict__ supp;
   const int *__restrict__ sym;
@@ -504,8 +538,10 @@ __global__ __launch_bounds__(NT) void k_badmm_setup(SetupArgs<T> a) {
       const int j = e / s, k = e - j * s;
       const T c = cost_raw<T, CM>(k, j, s, d, a.V, cs, xs, ids, a.dist);
       csum += fabs((double)c);
-      if (init) a.Z[base + e] = a.w[off + j] * a.cw[(size_t)l * s + k];
-      a.Y[base + e] = (T)0;
+      if (a.write_state) {  // the engine does this inside its first pass instead
+        if (init) a.Z[base + e] = a.w[off + j] * a.cw[(size_t)l * s + k];
+        a.Y[base + e] = (T)0;
+      }
     }
   }
   csum = block_sum(csum, red);
@@ -770,10 +806,10 @@ void launch_plan(int rmax, int G, size_t lds, hipStream_t S, const PlanArgs<T> &
   rmax = 0;
 #endif
   switch (rmax) {
-    case 8: k_badmm_plan<T, CM, 8, DIM><<<G, NT, lds, S>>>(pa, pa.perm, pa.p_str, pa.cum, pa.label); break;
-    case 16: k_badmm_plan<T, CM, 16, DIM><<<G, NT, lds, S>>>(pa, pa.perm, pa.p_str, pa.cum, pa.label); break;
-    case 32: k_badmm_plan<T, CM, 32, DIM><<<G, NT, lds, S>>>(pa, pa.perm, pa.p_str, pa.cum, pa.label); break;
-    default: k_badmm_plan<T, CM, 0, DIM><<<G, NT, lds, S>>>(pa, pa.perm, pa.p_str, pa.cum, pa.label); break;
+    case 8: k_badmm_plan<T, CM, 8, DIM><<<G, NT, lds, S>>>(pa, pa.perm, pa.p_str, pa.cum, pa.label, pa.lsw); break;
+    case 16: k_badmm_plan<T, CM, 16, DIM><<<G, NT, lds, S>>>(pa, pa.perm, pa.p_str, pa.cum, pa.label, pa.lsw); break;
+    case 32: k_badmm_plan<T, CM, 32, DIM><<<G, NT, lds, S>>>(pa, pa.perm, pa.p_str, pa.cum, pa.label, pa.lsw); break;
+    default: k_badmm_plan<T, CM, 0, DIM><<<G, NT, lds, S>>>(pa, pa.perm, pa.p_str, pa.cum, pa.label, pa.lsw); break;
   }
 }
 
@@ -838,6 +874,7 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
     sa.N = N; sa.s = s; sa.d = d; sa.V = p->V; sa.m_max = m_max; sa.per_block = setup_per_block;
     sa.p_str = p->p_str.as<int>(); sa.cum = p->cum.as<long long>(); sa.label = ctx->label.as<int>();
     sa.lsw = ctx->lsw.as<char>(); sa.force_init = fresh ? 1 : 0;
+    sa.write_state = variant == D2MI_BADMM_PLAIN ? 1 : 0;
     sa.w = w; sa.supp = supp; sa.sym = sym; sa.dist = dist; sa.cw = cwT; sa.csupp = csT;
     sa.Z = Z; sa.Y = Y; sa.part = ctx->part.as<double>();
     const size_t lds = align16(sizeof(T) * ((size_t)m_max * std::max(d, 1) + (size_t)s * std::max(d, 1)) +
@@ -913,6 +950,7 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
       pa.label = ctx->label.as<int>(); pa.w = w; pa.supp = supp; pa.sym = sym; pa.dist = dist;
       pa.cw = cwT; pa.csupp = csT; pa.scal = scal; pa.Z = Z; pa.Y = Y;
       pa.rec = ctx->rec.as<double>(); pa.resid_part = ctx->part.as<double>();
+      pa.first = iter == 0; pa.force_init = fresh ? 1 : 0; pa.lsw = ctx->lsw.as<char>();
       D2MI_HIP(hipEventRecord(evs[2 * iter], S));
       if constexpr (CM == COST_EUCLID) {
         if (d == 2) launch_plan<T, CM, 2>(rmax, G, lds, S, pa);
