@@ -42,6 +42,16 @@ class ClusterStats(C.Structure):
                 ("ms_labeling", C.c_double), ("ms_badmm", C.c_double)]
 
 
+class ServeOpts(C.Structure):
+    _fields_ = [("max_batch", C.c_int), ("streams", C.c_int), ("hbm_bytes", C.c_size_t),
+                ("max_m", C.c_int), ("use_triangle", C.c_int)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [(n, C.c_longlong) for n in ("submitted", "completed", "groups", "lps",
+                                             "memory_waits", "max_inflight_groups", "max_group")]
+
+
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
 _sp = C.POINTER(C.c_size_t)
@@ -74,6 +84,13 @@ SIGNATURES = {
     "d2mi_elkan_post": (C.c_int, [_vp, C.c_int]),
     "d2mi_clustering": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
                                   C.c_int, C.c_int, C.POINTER(ClusterStats)]),
+    "d2mi_serve_open": (C.c_int, [_vp, C.c_int, C.POINTER(ServeOpts), C.POINTER(_vp)]),
+    "d2mi_serve_submit": (C.c_int, [_vp, C.c_int, _dp, _dp, C.POINTER(C.c_longlong)]),
+    "d2mi_serve_progress": (C.c_int, [_vp, _ip]),
+    "d2mi_serve_result": (C.c_int, [_vp, C.c_longlong, _ip, _dp, _dp]),
+    "d2mi_serve_drain": (C.c_int, [_vp]),
+    "d2mi_serve_stats_get": (C.c_int, [_vp, C.POINTER(ServeStats)]),
+    "d2mi_serve_close": (C.c_int, [_vp]),
 }
 
 _lib = None

@@ -22,7 +22,6 @@
 namespace {
 
 constexpr int NT = 256;
-constexpr int LP_MAX_DIM = 8;  // Euclidean LP costs from supports in LDS for d <= 8
 
 enum Pairing { OBJ_CENT = 0, CENT_CENT = 1, OLD_NEW = 2 };
 
@@ -247,10 +246,6 @@ int lp_over_phases(d2mi_ctx *ctx, int selected_phase, Pairing pairing, int njobs
     a.status = ctx->lpstat.as<int>();
     switch (P.metric) {
       case D2MI_EUCLIDEAN:
-        if (P.dim > LP_MAX_DIM) {
-          d2mi_set_error("exact distances for d = %d > %d are not built yet", P.dim, LP_MAX_DIM);
-          return D2MI_EUNSUPPORTED;
-        }
         a.cost_mode = LP_COST_EUCLID;
         a.d = P.dim;
         break;
@@ -331,6 +326,21 @@ int elkan_reset_impl(d2mi_ctx *ctx, int K) {
 }
 
 }  // namespace
+
+// Exact W2 between every pair of resident centroids of the selected phases:
+// c_dev[a*K + b] (symmetric, zero diagonal), on the context stream.
+int d2mi_centroid_pairs(d2mi_ctx *ctx, int selected_phase, int K, double *c_dev, double *s_dev) {
+  D2MI_TRY(ctx->dist.ensure(sizeof(double) * (size_t)K * K));
+  D2MI_TRY(ctx->mask.ensure((size_t)K * K));
+  hipStream_t S = ctx->stream;
+  unsigned char *mask = ctx->mask.as<unsigned char>();
+  double *dist = ctx->dist.as<double>();
+  k_cc_mask<<<nblk((size_t)K * K), NT, 0, S>>>(mask, K);
+  D2MI_TRY(lp_over_phases(ctx, selected_phase, CENT_CENT, K * K, K, mask, nullptr, nullptr, dist));
+  k_cc_fill<<<nblk(K), NT, 0, S>>>(dist, K, c_dev, s_dev);
+  D2MI_HIP(hipGetLastError());
+  return lp_status(ctx);
+}
 
 // ------------------------------------------------------------------ C ABI
 

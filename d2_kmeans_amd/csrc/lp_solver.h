@@ -1,0 +1,503 @@
+// Exact transport LP, one workgroup per problem (device code shared by the
+// batched-distance kernels and the assignment server).
+//
+// The LP is the one the reference hands to Mosek for every distance
+// (d2_match_by_distmat, solver_mosek.cc:61-143):
+//     min sum_{k,j} C[k + j*n] x_kj,  x >= 0,
+//     sum_j x_kj = wA[k] (all n rows),  sum_k x_kj = wB[j] (j < m-1; the last
+//     column constraint is dropped, so column m-1 absorbs the remainder).
+//
+// Transportation simplex in FP64 on a spanning-tree basis (parent pointers,
+// root = row 0), run by all waves of the workgroup:
+//   * start: north-west corner;
+//   * potentials, depths and the binary-lifting table lvl[b][v] (the 2^b-th
+//     ancestor of v) come out of one pointer-jumping pass per pivot: each
+//     round doubles the reach of every node, so a tree of depth D costs
+//     ceil(log2 D)+1 rounds;
+//   * pricing: block search (the next block of T*cpt cells after the last
+//     entering cell; the most negative reduced cost in the first block that
+//     has one); after 50*(n+m)+1000 pivots, Bland's rule (lowest improving
+//     cell, lowest leaving cell on ties), which cannot cycle;
+//   * pivot: every wave finds the apex of the cycle by testing 64 depths at a
+//     time against the lifting table; path nodes, the ratio test, the flow
+//     update and the re-hanging of the cut subtree (parent pointers reversed
+//     along the path) are then parallel over path positions.
+// Costs: an FP32 copy of the n*m cost matrix sits in LDS when it fits and
+// drives the pivots.  At optimality for the FP32 costs the solver switches to
+// the exact FP64 costs: potentials are recomputed from them and every cell
+// whose FP32 reduced cost is within the table's error bound of zero is priced
+// exactly; pivoting continues on exact costs until no cell improves.  The
+// returned value sums exact FP64 costs over the final basis, so it does not
+// depend on the FP32 table.
+#pragma once
+
+#include <cfloat>
+#include <climits>
+#include <cmath>
+
+namespace lp {
+
+constexpr int WAVE = 64;
+constexpr int MAX_WAVES = 16;
+
+enum CostMode {
+  COST_EUCLID = 0,  // squared Euclidean distance of supports (A rows, B columns), FP64
+  COST_HIST = 1,    // dense histogram ground cost dist[k + j*V]
+  COST_SPARSE = 2,  // sparse histogram: dist[ids[j]*V + k]
+  COST_GLOBAL = 3,  // precomputed FP64 costs g[k + j*n]
+};
+
+enum TableMode {
+  TAB_NONE = 0,      // price every cell with the exact cost
+  TAB_EXACT = 1,     // LDS table = FP32 rounding of the exact cost
+  TAB_GLOBAL32 = 2,  // LDS table copied from a precomputed FP32 table gt[k + j*n]
+};
+
+__host__ __device__ inline size_t al16(size_t x) { return (x + 15) & ~size_t(15); }
+
+__host__ __device__ inline int ceil_log2(int x) {
+  int r = 0;
+  while ((1 << r) < x) ++r;
+  return r;
+}
+
+// LDS layout for problems up to n_max x m_max; offsets in bytes.
+struct Layout {
+  size_t ct, A, B, wa, wb, ids, accA, accB, flow, par, depA, depB, lvl, path, ctl, total;
+  int levels;
+  __host__ __device__ Layout(int n_max, int m_max, int dA, int dB, bool table, bool ids_on) {
+    const size_t nn = (size_t)n_max + m_max;
+    levels = ceil_log2((int)nn) + 2;
+    size_t o = 0;
+    ct = o; o = al16(o + (table ? sizeof(float) * (size_t)n_max * m_max : 0));
+    A = o; o = al16(o + sizeof(double) * (size_t)n_max * dA);
+    B = o; o = al16(o + sizeof(double) * (size_t)m_max * dB);
+    wa = o; o = al16(o + sizeof(double) * n_max);
+    wb = o; o = al16(o + sizeof(double) * m_max);
+    ids = o; o = al16(o + (ids_on ? sizeof(int) * m_max : 0));
+    accA = o; o = al16(o + sizeof(double) * nn);
+    accB = o; o = al16(o + sizeof(double) * nn);
+    flow = o; o = al16(o + sizeof(double) * nn);
+    par = o; o = al16(o + sizeof(int) * nn);
+    depA = o; o = al16(o + sizeof(int) * nn);
+    depB = o; o = al16(o + sizeof(int) * nn);
+    lvl = o; o = al16(o + sizeof(short) * nn * levels);
+    path = o; o = al16(o + sizeof(int) * (nn + 1));
+    ctl = o; o = al16(o + 1024);
+    total = o;
+  }
+};
+
+struct Cost {
+  int mode, d, n, V;
+  const double *A, *B;  // Euclidean supports: A row k at A + k*d, B column j at B + j*d
+  const int *ids;       // sparse-histogram bin ids (LDS)
+  const double *__restrict__ dist;
+  const double *__restrict__ g;
+  __device__ __forceinline__ double operator()(int k, int j) const {
+    switch (mode) {
+      case COST_EUCLID: {
+        const double *a = A + (size_t)k * d, *b = B + (size_t)j * d;
+        double acc = 0.0;
+        for (int t = 0; t < d; ++t) {
+          const double u = a[t] - b[t];
+          acc += u * u;
+        }
+        return acc;
+      }
+      case COST_HIST:
+        return dist[k + (size_t)j * V];
+      case COST_SPARSE:
+        return dist[(size_t)ids[j] * V + k];
+      default:
+        return g[k + (size_t)j * n];
+    }
+  }
+};
+
+// e = k + j*n  ->  (k, j) without an integer divide (e < 2^24)
+__device__ __forceinline__ void split_cell(int e, int n, float inv_n, int &k, int &j) {
+  int q = (int)((float)e * inv_n);
+  int r = e - q * n;
+  if (r < 0) { --q; r += n; }
+  else if (r >= n) { ++q; r -= n; }
+  k = r;
+  j = q;
+}
+
+__device__ __forceinline__ double wave_min_d(double v) {
+  for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, WAVE));
+  return v;
+}
+__device__ __forceinline__ double wave_sum_d(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
+  return v;
+}
+__device__ __forceinline__ int wave_min_i(int v) {
+  for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, WAVE));
+  return v;
+}
+__device__ __forceinline__ double wave_max_d(double v) {
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, WAVE));
+  return v;
+}
+
+// Control block in LDS: reduction slots (two banks, alternated per use so a
+// slot is never rewritten before every wave has read it) and flags.
+struct Ctl {
+  double rv[2][MAX_WAVES];
+  int ri[2][MAX_WAVES];
+  int flag[2][MAX_WAVES];
+};
+
+// Workgroup argmin of (v, i) lexicographically; every thread gets the result.
+__device__ __forceinline__ void block_argmin(double &v, int &i, Ctl *ctl, int &bank, int nw) {
+  const double wv = wave_min_d(v);
+  const int wi = wave_min_i(v == wv ? i : INT_MAX);
+  const int wid = threadIdx.x / WAVE;
+  if ((threadIdx.x & (WAVE - 1)) == 0) {
+    ctl->rv[bank][wid] = wv;
+    ctl->ri[bank][wid] = wi;
+  }
+  __syncthreads();
+  double bv = ctl->rv[bank][0];
+  int bi = ctl->ri[bank][0];
+  for (int w = 1; w < nw; ++w) {
+    const double x = ctl->rv[bank][w];
+    const int y = ctl->ri[bank][w];
+    if (x < bv || (x == bv && y < bi)) { bv = x; bi = y; }
+  }
+  bank ^= 1;
+  v = bv;
+  i = bi;
+}
+
+__device__ __forceinline__ double block_sum(double v, Ctl *ctl, int &bank, int nw) {
+  v = wave_sum_d(v);
+  const int wid = threadIdx.x / WAVE;
+  if ((threadIdx.x & (WAVE - 1)) == 0) ctl->rv[bank][wid] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int w = 0; w < nw; ++w) s += ctl->rv[bank][w];
+  bank ^= 1;
+  return s;
+}
+
+__device__ __forceinline__ double block_max(double v, Ctl *ctl, int &bank, int nw) {
+  v = wave_max_d(v);
+  const int wid = threadIdx.x / WAVE;
+  if ((threadIdx.x & (WAVE - 1)) == 0) ctl->rv[bank][wid] = v;
+  __syncthreads();
+  double s = ctl->rv[bank][0];
+  for (int w = 1; w < nw; ++w) s = fmax(s, ctl->rv[bank][w]);
+  bank ^= 1;
+  return s;
+}
+
+__device__ __forceinline__ int block_or(int f, Ctl *ctl, int &bank, int nw) {
+  f = __any(f) ? 1 : 0;
+  const int wid = threadIdx.x / WAVE;
+  if ((threadIdx.x & (WAVE - 1)) == 0) ctl->flag[bank][wid] = f;
+  __syncthreads();
+  int s = 0;
+  for (int w = 0; w < nw; ++w) s |= ctl->flag[bank][w];
+  bank ^= 1;
+  return s;
+}
+
+struct Problem {
+  int n, m;
+  const double *wa, *wb;  // LDS
+  Cost C;
+  int table;              // TableMode
+  const float *gt;        // TAB_GLOBAL32 source (global)
+  float tab_rel;          // |table - exact| <= tab_rel * |table| + 1e-37
+  int cpt;                // cells per thread per pricing block
+};
+
+// Solves one LP with the whole workgroup (blockDim.x a multiple of 64, at
+// most 64*MAX_WAVES threads); returns the optimal value to all threads.
+// status bit 0 is set when the pivot limit is reached.
+__device__ double solve(const Problem &P, unsigned char *lds, const Layout &L, int *pivots_out,
+                        int *status) {
+  const int tid = threadIdx.x, T = blockDim.x, nw = T / WAVE;
+  const int lane = tid & (WAVE - 1);
+  const int n = P.n, m = P.m, nn = n + m, N = n * m;
+  const Cost &C = P.C;
+  const bool table = P.table != TAB_NONE;
+  float *ct = (float *)(lds + L.ct);
+  double *accA = (double *)(lds + L.accA), *accB = (double *)(lds + L.accB);
+  double *flow = (double *)(lds + L.flow);
+  int *par = (int *)(lds + L.par), *depA = (int *)(lds + L.depA), *depB = (int *)(lds + L.depB);
+  short *lvl = (short *)(lds + L.lvl);
+  int *path = (int *)(lds + L.path);
+  Ctl *ctl = (Ctl *)(lds + L.ctl);
+  int bank = 0;
+  const float inv_n = 1.0f / (float)n;
+
+  // ---- cost table and the cost scale for the reduced-cost tolerance
+  double cmax = 0.0;
+  if (table) {
+    if (P.table == TAB_GLOBAL32) {
+      for (int e = tid; e < N; e += T) {
+        const float c = P.gt[e];
+        ct[e] = c;
+        cmax = fmax(cmax, fabs((double)c));
+      }
+    } else {
+      for (int e = tid; e < N; e += T) {
+        int k, j;
+        split_cell(e, n, inv_n, k, j);
+        const double c = C(k, j);
+        ct[e] = (float)c;
+        cmax = fmax(cmax, fabs(c));
+      }
+    }
+    cmax *= 1.0 + (double)P.tab_rel;
+  } else {
+    for (int e = tid; e < N; e += T) {
+      int k, j;
+      split_cell(e, n, inv_n, k, j);
+      cmax = fmax(cmax, fabs(C(k, j)));
+    }
+  }
+  cmax = block_max(cmax, ctl, bank, nw);
+  const double eps = 1e-12 * (1.0 + cmax);
+
+  // ---- north-west corner: each new cell brings one new node, hung under
+  // the endpoint already in the tree; flow[v] = flow on v's parent edge
+  if (tid == 0) {
+    double sa = 0.0, sbm = 0.0;
+    for (int k = 0; k < n; ++k) { accA[k] = P.wa[k]; sa += P.wa[k]; }
+    for (int j = 0; j < m - 1; ++j) { accB[j] = P.wb[j]; sbm += P.wb[j]; }
+    accB[m - 1] = sa - sbm > 0.0 ? sa - sbm : 0.0;
+    par[0] = -1;
+    flow[0] = 0.0;
+    int i = 0, j = 0, newnode = n;  // the first cell (0,0) brings column 0
+    for (int t = 0; t < nn - 1; ++t) {
+      double q = fmin(accA[i], accB[j]);
+      if (q < 0.0) q = 0.0;
+      par[newnode] = newnode >= n ? i : n + j;
+      flow[newnode] = q;
+      accA[i] -= q;
+      accB[j] -= q;
+      if (t == nn - 2) break;
+      if (i == n - 1) { ++j; newnode = n + j; }
+      else if (j == m - 1) { ++i; newnode = i; }
+      else if (accA[i] <= accB[j]) { ++i; newnode = i; }
+      else { ++j; newnode = n + j; }
+    }
+  }
+  __syncthreads();
+
+  // cost of the tree edge at node v (edge to parent p), table or exact
+  auto edge_cell = [&](int v, int p) { return v < n ? v + (p - n) * n : p + (v - n) * n; };
+  auto edge_k = [&](int v, int p) { return v < n ? v : p; };
+  auto edge_j = [&](int v, int p) { return v < n ? p - n : v - n; };
+
+  bool exact_mode = !table;
+  int pivots = 0, pos0 = 0;
+  const int blk = T * P.cpt;
+  const int dantzig_limit = 50 * nn + 1000;
+  const int hard_limit = 200000 + 100 * N;
+  double *pot = accA;
+  int *dep = depA;
+  for (;;) {
+    // ---- pointer jumping: potentials, depths, lifting table
+    for (int v = tid; v < nn; v += T) {
+      const int p = par[v];
+      lvl[v] = (short)p;
+      double c = 0.0;
+      if (p >= 0) c = exact_mode ? C(edge_k(v, p), edge_j(v, p)) : (double)ct[edge_cell(v, p)];
+      accA[v] = c;
+      depA[v] = p >= 0 ? 1 : 0;
+    }
+    __syncthreads();
+    {
+      double *aS = accA, *aD = accB;
+      int *dS = depA, *dD = depB;
+      for (int r = 0; r + 1 < L.levels; ++r) {
+        const short *cur = lvl + (size_t)r * nn;
+        short *nxt = lvl + (size_t)(r + 1) * nn;
+        const double sg = r == 0 ? -1.0 : 1.0;
+        int any = 0;
+        for (int v = tid; v < nn; v += T) {
+          const int a = cur[v];
+          if (a >= 0) {
+            const int aa = cur[a];
+            aD[v] = aS[v] + sg * aS[a];
+            dD[v] = dS[v] + dS[a];
+            nxt[v] = (short)aa;
+            any |= aa >= 0;
+          } else {
+            aD[v] = aS[v];
+            dD[v] = dS[v];
+            nxt[v] = -1;
+          }
+        }
+        any = block_or(any, ctl, bank, nw);
+        double *td = aS; aS = aD; aD = td;
+        int *ti = dS; dS = dD; dD = ti;
+        if (!any) break;
+      }
+      pot = aS;
+      dep = dS;
+    }
+    double *pfl = pot == accA ? accB : accA;  // free during the pivot
+
+    // ---- pricing
+    const bool bland = pivots >= dantzig_limit;
+    int ent = INT_MAX;
+    double ent_r = 0.0;
+    for (int scanned = 0; scanned < N; scanned += blk) {
+      int base = (bland ? 0 : pos0) + scanned;
+      if (base >= N) base -= N;
+      double best = -eps;
+      int bestc = INT_MAX;
+      for (int i = 0; i < P.cpt; ++i) {
+        const int off = tid + i * T;
+        if (scanned + off >= N) break;
+        int e = base + off;
+        if (e >= N) e -= N;
+        int k, j;
+        split_cell(e, n, inv_n, k, j);
+        const double pu = pot[k], pv = pot[n + j];
+        double r;
+        if (!table) {
+          r = C(k, j) - pu - pv;
+        } else {
+          const double c = (double)ct[e];
+          r = c - pu - pv;
+          if (exact_mode) {
+            if (r < (double)P.tab_rel * fabs(c) + 1e-37 + eps) r = C(k, j) - pu - pv;
+            else continue;
+          }
+        }
+        if (bland) {
+          if (r < -eps && e < bestc) { bestc = e; best = r; }
+        } else if (r < best || (r == best && bestc != INT_MAX && e < bestc)) {
+          best = r;
+          bestc = e;
+        }
+      }
+      if (bland) {
+        double dummy = bestc == INT_MAX ? 0.0 : -1.0;
+        block_argmin(dummy, bestc, ctl, bank, nw);
+        if (bestc != INT_MAX) { ent = bestc; break; }
+      } else {
+        block_argmin(best, bestc, ctl, bank, nw);
+        if (bestc != INT_MAX && best < -eps) {
+          ent = bestc;
+          ent_r = best;
+          pos0 = base + blk;
+          while (pos0 >= N) pos0 -= N;
+          break;
+        }
+      }
+    }
+    (void)ent_r;
+    if (ent == INT_MAX) {
+      if (!exact_mode) {  // optimal for the FP32 table: verify on exact costs
+        exact_mode = true;
+        continue;
+      }
+      break;
+    }
+    if (pivots >= hard_limit) {
+      if (tid == 0) atomicOr(status, 1);
+      break;
+    }
+    int ei, ej;
+    split_cell(ent, n, inv_n, ei, ej);
+    const int a = n + ej, b = ei;
+    const int da = dep[a], db = dep[b];
+
+    // ---- apex of the cycle: largest depth D with anc(a, da-D) == anc(b, db-D)
+    auto anc = [&](int v, int k) {
+      for (int bit = 0; k; ++bit, k >>= 1)
+        if (k & 1) v = lvl[(size_t)bit * nn + v];
+      return v;
+    };
+    int Dstar = 0;
+    for (int top = min(da, db); top >= 0; top -= WAVE) {
+      const int D = top - lane;
+      const bool eq = D >= 0 && anc(a, da - D) == anc(b, db - D);
+      const unsigned long long bal = __ballot(eq);
+      if (bal) {
+        Dstar = top - (__ffsll((long long)bal) - 1);
+        break;
+      }
+    }
+    const int nA = da - Dstar, nB = db - Dstar, np = nA + nB;
+
+    // ---- path nodes and the ratio test over the minus edges (even t)
+    double th = DBL_MAX;
+    int thk = INT_MAX;
+    for (int q = tid; q < np; q += T) {
+      const bool sideB = q >= nA;
+      const int t = sideB ? q - nA : q;
+      const int v = anc(sideB ? b : a, t);
+      path[q] = v;
+      const double f = flow[v];
+      pfl[q] = f;
+      if ((t & 1) == 0) {
+        const int key = bland ? edge_cell(v, par[v]) : q;
+        if (f < th || (f == th && key < thk)) { th = f; thk = key; }
+      }
+    }
+    block_argmin(th, thk, ctl, bank, nw);
+    // the leaving position: recover it from the key
+    int qpos = thk;
+    if (bland) {
+      // the key is a cell index: find the position holding it
+      int found = INT_MAX;
+      for (int q = tid; q < np; q += T) {
+        const bool sideB = q >= nA;
+        const int t = sideB ? q - nA : q;
+        if ((t & 1) == 0) {
+          const int v = path[q];
+          if (pfl[q] == th && edge_cell(v, par[v]) == thk) found = min(found, q);
+        }
+      }
+      double dummy = 0.0;
+      block_argmin(dummy, found, ctl, bank, nw);
+      qpos = found;
+    }
+    const double theta = th;
+    const int qside = qpos >= nA ? 1 : 0;
+    const int qt = qside ? qpos - nA : qpos;
+
+    // ---- flows around the cycle and the re-hang of the cut subtree
+    for (int q = tid; q < np; q += T) {
+      const int side = q >= nA ? 1 : 0;
+      const int t = side ? q - nA : q;
+      const int v = path[q];
+      if (side == qside && t <= qt) {
+        if (t == 0) {
+          par[v] = side == 0 ? b : a;
+          flow[v] = theta;
+        } else {
+          par[v] = path[q - 1];
+          flow[v] = pfl[q - 1] + (((t - 1) & 1) ? theta : -theta);
+        }
+      } else {
+        flow[v] = pfl[q] + ((t & 1) ? theta : -theta);
+      }
+    }
+    __syncthreads();
+    ++pivots;
+  }
+
+  // ---- objective on exact costs over the tree edges
+  double val = 0.0;
+  for (int v = tid; v < nn; v += T) {
+    const int p = par[v];
+    if (p >= 0) val += flow[v] * C(edge_k(v, p), edge_j(v, p));
+  }
+  val = block_sum(val, ctl, bank, nw);
+  if (pivots_out) *pivots_out = pivots;
+  __syncthreads();
+  return val;
+}
+
+}  // namespace lp

@@ -39,12 +39,19 @@ struct LpArgs {
   const double *gcost;
   long long gstride;
   int n_max, m_max;
+  // launch shape, filled by d2mi_plan_transport
+  int threads, cpt, table, stage_a, stage_b;
+  size_t lds;
+  int threads_hint;     // > 0: workgroup size to use
+  hipStream_t stream;   // null: the context stream
   double *out;
   int *pivots;  // may be null
   int *status;  // bit 0: an LP hit the pivot limit
 };
 
-size_t d2mi_lp_lds_bytes(int n_max, int m_max, int dsupp);
+int d2mi_plan_transport(d2mi_ctx *ctx, LpArgs &a);
+// Exact W2 between all pairs of resident centroids (labeling.hip).
+int d2mi_centroid_pairs(d2mi_ctx *ctx, int selected_phase, int K, double *c_dev, double *s_dev);
 int d2mi_launch_transport(d2mi_ctx *ctx, const LpArgs &a);
 int d2mi_zero_doubles(d2mi_ctx *ctx, double *p, int n);
 int d2mi_zero_ints(d2mi_ctx *ctx, int *p, int n);

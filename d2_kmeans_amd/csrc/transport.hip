@@ -1,282 +1,23 @@
-// Exact transport LP values on the device: the problem the reference hands to
-// Mosek's network simplex for every distance (d2_match_by_distmat,
-// solver_mosek.cc:61-143):
-//     min sum_{k,j} C[k + j*n] x_kj,  x >= 0,
-//     sum_j x_kj = wA[k] (all n rows),  sum_k x_kj = wB[j] (j < m-1; the last
-//     column constraint is dropped, so column m-1 absorbs the remainder).
-//
-// One 64-lane wavefront solves one LP with the transportation simplex in
-// FP64:
-//   * start: north-west corner, which is a spanning tree of n+m-1 cells;
-//   * pricing: every lane scans a strided share of the n*m cells for the most
-//     negative reduced cost C - u_k - v_j (Dantzig), ties to the lowest cell
-//     index; after many pivots the rule switches to Bland's first-negative
-//     cell, which cannot cycle;
-//   * pivot: lane 0 walks the basis tree (parent pointers) to find the cycle,
-//     moves theta around it and re-hangs the cut subtree by reversing the
-//     parent pointers on the path to the leaving edge;
-//   * potentials u, v: recomputed for every node by pointer jumping, so a
-//     pivot costs O(log depth) parallel rounds, not a sequential tree walk.
-// Costs come from supports held in LDS (Euclidean, squared distance by
-// direct differences), from a histogram ground-cost matrix, or from a
-// precomputed FP64 cost array.
-#include <cfloat>
-#include <cmath>
+// Batched exact transport LPs on the device: one workgroup per LP, the
+// solver in lp_solver.h.  Costs come from supports staged in LDS or read
+// from HBM (Euclidean, squared distance by direct differences), from a
+// histogram ground-cost matrix, or from a precomputed FP64 cost array.
+#include <algorithm>
+#include <cstdlib>
 
 #include "d2mi_internal.h"
+#include "lp_solver.h"
 #include "transport.h"
 
 namespace {
 
-constexpr int WAVE = 64;
-
-__host__ __device__ inline size_t al16(size_t x) { return (x + 15) & ~size_t(15); }
-
-struct LpLds {
-  size_t A, B, wa, wb, ids, pot, flow, acc0, acc1, par, anc0, anc1, sg0, sg1, mark, pA, pB, total;
-  __host__ __device__ LpLds(int n_max, int m_max, int dsupp) {
-    const size_t nn = (size_t)n_max + m_max;
-    size_t o = 0;
-    A = o; o = al16(o + sizeof(double) * (size_t)n_max * dsupp);
-    B = o; o = al16(o + sizeof(double) * (size_t)m_max * dsupp);
-    wa = o; o = al16(o + sizeof(double) * n_max);
-    wb = o; o = al16(o + sizeof(double) * m_max);
-    ids = o; o = al16(o + sizeof(int) * m_max);
-    pot = o; o = al16(o + sizeof(double) * nn);
-    flow = o; o = al16(o + sizeof(double) * nn);
-    acc0 = o; o = al16(o + sizeof(double) * nn);
-    acc1 = o; o = al16(o + sizeof(double) * nn);
-    par = o; o = al16(o + sizeof(int) * nn);
-    anc0 = o; o = al16(o + sizeof(int) * nn);
-    anc1 = o; o = al16(o + sizeof(int) * nn);
-    sg0 = o; o = al16(o + sizeof(int) * nn);
-    sg1 = o; o = al16(o + sizeof(int) * nn);
-    mark = o; o = al16(o + sizeof(int) * nn);
-    pA = o; o = al16(o + sizeof(int) * (nn + 1));
-    pB = o; o = al16(o + sizeof(int) * (nn + 1));
-    total = o;
-  }
-};
-
-__device__ __forceinline__ double wmin_d(double v) {
-  for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, WAVE));
-  return v;
-}
-__device__ __forceinline__ double wsum_d(double v) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
-  return v;
-}
-__device__ __forceinline__ int wmin_i(int v) {
-  for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, WAVE));
-  return v;
-}
-
-struct Cost {
-  int mode, d, n, V;
-  const double *A, *B;       // LDS supports (Euclidean)
-  const int *ids;            // LDS bin ids (sparse histogram)
-  const double *__restrict__ dist;  // ground cost [V*V]
-  const double *__restrict__ g;     // precomputed [n*m], element (k,j) at k + j*n
-  __device__ __forceinline__ double operator()(int k, int j) const {
-    switch (mode) {
-      case LP_COST_EUCLID: {
-        double acc = 0.0;
-        for (int t = 0; t < d; ++t) {
-          const double u = A[k * d + t] - B[j * d + t];
-          acc += u * u;
-        }
-        return acc;
-      }
-      case LP_COST_HIST:
-        return dist[k + (size_t)j * V];
-      case LP_COST_SPARSE:
-        return dist[(size_t)ids[j] * V + k];
-      default:
-        return g[k + (size_t)j * n];
-    }
-  }
-};
-
-// cost of the tree edge stored at node v (edge to its parent)
-__device__ __forceinline__ double edge_cost(const Cost &C, int v, int p, int n) {
-  return v < n ? C(v, p - n) : C(p, v - n);
-}
-
-// Solves one LP with the wave; returns the optimal value to all lanes.
-__device__ double solve_lp(const Cost &C, int n, int m, const double *wa, const double *wb,
-                           unsigned char *lds, const LpLds &L, int *pivots_out, int *status) {
-  const int lane = threadIdx.x;
-  const int nn = n + m;
-  double *pot = (double *)(lds + L.pot), *flow = (double *)(lds + L.flow);
-  double *acc0 = (double *)(lds + L.acc0), *acc1 = (double *)(lds + L.acc1);
-  int *par = (int *)(lds + L.par), *anc0 = (int *)(lds + L.anc0), *anc1 = (int *)(lds + L.anc1);
-  int *sg0 = (int *)(lds + L.sg0), *sg1 = (int *)(lds + L.sg1), *mark = (int *)(lds + L.mark);
-  int *pA = (int *)(lds + L.pA), *pB = (int *)(lds + L.pB);
-
-  // cost scale for the reduced-cost tolerance
-  double cmax = 0.0;
-  for (int e = lane; e < n * m; e += WAVE) cmax = fmax(cmax, fabs(C(e % n, e / n)));
-  cmax = -wmin_d(-cmax);
-  const double eps = 1e-12 * (1.0 + cmax);
-
-  for (int v = lane; v < nn; v += WAVE) mark[v] = 0;
-  __syncthreads();
-
-  // north-west corner: each new cell brings one new node, hung under the
-  // endpoint already in the tree; flow[v] is the flow on v's parent edge
-  if (lane == 0) {
-    double sa = 0.0, sbm = 0.0;
-    for (int k = 0; k < n; ++k) { acc0[k] = wa[k]; sa += wa[k]; }
-    for (int j = 0; j < m - 1; ++j) { acc1[j] = wb[j]; sbm += wb[j]; }
-    acc1[m - 1] = sa - sbm > 0.0 ? sa - sbm : 0.0;
-    par[0] = -1;
-    flow[0] = 0.0;
-    int i = 0, j = 0, newnode = n;  // first cell (0,0) introduces column 0
-    for (int t = 0; t < n + m - 1; ++t) {
-      double q = fmin(acc0[i], acc1[j]);
-      if (q < 0.0) q = 0.0;
-      if (newnode >= n) par[newnode] = i;       // new column hangs under row i
-      else par[newnode] = n + j;                // new row hangs under column j
-      flow[newnode] = q;
-      acc0[i] -= q;
-      acc1[j] -= q;
-      if (t == n + m - 2) break;
-      if (i == n - 1) { ++j; newnode = n + j; }
-      else if (j == m - 1) { ++i; newnode = i; }
-      else if (acc0[i] <= acc1[j]) { ++i; newnode = i; }
-      else { ++j; newnode = n + j; }
-    }
-  }
-  __syncthreads();
-
-  int pivots = 0;
-  const int dantzig_limit = 50 * nn + 1000;
-  const int hard_limit = 200000 + 100 * n * m;
-  for (;;) {
-    // ---- potentials by pointer jumping: pot[v] = acc[v] + sg[v] * pot[anc[v]]
-    for (int v = lane; v < nn; v += WAVE) {
-      const int p = par[v];
-      anc0[v] = p;
-      acc0[v] = p >= 0 ? edge_cost(C, v, p, n) : 0.0;
-      sg0[v] = p >= 0 ? -1 : 0;
-    }
-    __syncthreads();
-    int *ancA = anc0, *ancB = anc1, *sgA = sg0, *sgB = sg1;
-    double *accA = acc0, *accB = acc1;
-    for (;;) {
-      int any = 0;
-      for (int v = lane; v < nn; v += WAVE) {
-        const int a = ancA[v];
-        if (a >= 0) {
-          accB[v] = accA[v] + sgA[v] * accA[a];
-          sgB[v] = sgA[v] * sgA[a];
-          ancB[v] = ancA[a];
-          any |= ancA[a] >= 0;
-        } else {
-          accB[v] = accA[v];
-          sgB[v] = sgA[v];
-          ancB[v] = -1;
-        }
-      }
-      __syncthreads();
-      int *ti = ancA; ancA = ancB; ancB = ti;
-      ti = sgA; sgA = sgB; sgB = ti;
-      double *td = accA; accA = accB; accB = td;
-      any = __any(any);
-      if (!any) break;
-    }
-    for (int v = lane; v < nn; v += WAVE) pot[v] = accA[v];
-    __syncthreads();
-
-    // ---- pricing
-    const bool bland = pivots >= dantzig_limit;
-    double best = -eps;
-    int bestc = 0x7fffffff;
-    for (int e = lane; e < n * m; e += WAVE) {
-      const int k = e % n, j = e / n;
-      const double r = C(k, j) - pot[k] - pot[n + j];
-      if (bland) {
-        if (r < -eps && e < bestc) { bestc = e; best = r; }
-      } else if (r < best || (r == best && e < bestc && r < -eps)) {
-        best = r;
-        bestc = e;
-      }
-    }
-    if (bland) {
-      const int c = wmin_i(bestc);
-      if (c == 0x7fffffff) break;
-      bestc = c;
-    } else {
-      const double bm = wmin_d(best);
-      const int c = wmin_i(best == bm ? bestc : 0x7fffffff);
-      if (!(bm < -eps) || c == 0x7fffffff) break;
-      best = bm;
-      bestc = c;
-    }
-    if (pivots >= hard_limit) {
-      if (lane == 0) atomicOr(status, 1);
-      break;
-    }
-    const int ei = bestc % n, ej = bestc / n;
-
-    // ---- pivot (lane 0): cycle = tree path from column ej to row ei
-    if (lane == 0) {
-      const int a = n + ej, b = ei;
-      int la = 0;
-      for (int v = a; v >= 0; v = par[v]) { pA[la] = v; mark[v] = la + 1; ++la; }
-      int lb = 0, v = b;
-      while (mark[v] == 0) { pB[lb++] = v; v = par[v]; }
-      const int nA = mark[v] - 1;  // edges on a's side: pA[0..nA)
-      for (int t = 0; t < la; ++t) mark[pA[t]] = 0;
-      // minus edges: even positions from each end
-      double theta = DBL_MAX;
-      int qside = -1, qt = -1, qnode = 0x7fffffff;
-      for (int t = 0; t < nA; t += 2) {
-        const double f = flow[pA[t]];
-        if (f < theta || (bland && f == theta && pA[t] < qnode)) {
-          theta = f; qside = 0; qt = t; qnode = pA[t];
-        }
-      }
-      for (int t = 0; t < lb; t += 2) {
-        const double f = flow[pB[t]];
-        if (f < theta || (bland && f == theta && pB[t] < qnode)) {
-          theta = f; qside = 1; qt = t; qnode = pB[t];
-        }
-      }
-      for (int t = 0; t < nA; ++t) flow[pA[t]] += (t & 1) ? theta : -theta;
-      for (int t = 0; t < lb; ++t) flow[pB[t]] += (t & 1) ? theta : -theta;
-      // re-hang: the cut subtree holds the path pX[0..qt]; reverse it and
-      // hang its end pX[0] (a or b) under the entering cell's other end
-      int *P = qside == 0 ? pA : pB;
-      const int other = qside == 0 ? b : a;
-      for (int t = qt; t >= 1; --t) {
-        par[P[t]] = P[t - 1];
-        flow[P[t]] = flow[P[t - 1]];
-      }
-      par[P[0]] = other;
-      flow[P[0]] = theta;
-    }
-    __syncthreads();
-    ++pivots;
-  }
-
-  // objective: sum over tree edges
-  double val = 0.0;
-  for (int v = lane; v < nn; v += WAVE) {
-    const int p = par[v];
-    if (p >= 0) val += flow[v] * edge_cost(C, v, p, n);
-  }
-  val = wsum_d(val);
-  if (pivots_out) *pivots_out = pivots;
-  __syncthreads();
-  return val;
-}
-
-__global__ __launch_bounds__(WAVE) void k_transport(LpArgs a) {
+__global__ __launch_bounds__(1024) void k_transport(LpArgs a) {
   extern __shared__ __align__(16) unsigned char lds[];
-  const int lane = threadIdx.x;
-  const int dsupp = a.cost_mode == LP_COST_EUCLID ? a.d : 0;
-  const LpLds L(a.n_max, a.m_max, dsupp > 0 ? dsupp : 1);
+  const int tid = threadIdx.x, T = blockDim.x;
+  const bool euclid = a.cost_mode == LP_COST_EUCLID;
+  const int dA = euclid && a.stage_a ? a.d : 0, dB = euclid && a.stage_b ? a.d : 0;
+  const lp::Layout L(a.n_max, a.m_max, dA, dB, a.table != lp::TAB_NONE,
+                     a.cost_mode == LP_COST_SPARSE);
   double *A = (double *)(lds + L.A), *B = (double *)(lds + L.B);
   double *wa = (double *)(lds + L.wa), *wb = (double *)(lds + L.wb);
   int *ids = (int *)(lds + L.ids);
@@ -286,43 +27,59 @@ __global__ __launch_bounds__(WAVE) void k_transport(LpArgs a) {
     const int ka = a.implicit_K > 0 ? job % a.implicit_K : a.job_a[job];
     const int jb = a.implicit_K > 0 ? job / a.implicit_K : a.job_b[job];
     const int n = a.sA;
-    const double *Asrc = a.A_supp ? a.A_supp + (size_t)ka * n * dsupp : nullptr;
+    const double *Asrc = euclid ? a.A_supp + (size_t)ka * n * a.d : nullptr;
     const double *wA = a.A_w + (size_t)ka * n;
     int m;
     const double *Bsrc = nullptr, *wB;
     const int *idsrc = nullptr;
     if (a.b_is_table) {
       m = a.sB;
-      Bsrc = a.B_supp ? a.B_supp + (size_t)jb * m * dsupp : nullptr;
+      Bsrc = euclid ? a.B_supp + (size_t)jb * m * a.d : nullptr;
       wB = a.B_w + (size_t)jb * m;
     } else {
       m = a.p_str[jb];
       const long long off = a.cum[jb];
-      Bsrc = a.supp ? a.supp + (size_t)off * dsupp : nullptr;
+      Bsrc = euclid ? a.supp + (size_t)off * a.d : nullptr;
       wB = a.w + off;
       idsrc = a.sym ? a.sym + off : nullptr;
     }
     __syncthreads();
-    for (int q = lane; q < n; q += WAVE) wa[q] = wA[q];
-    for (int q = lane; q < m; q += WAVE) wb[q] = wB[q];
-    if (dsupp > 0) {
-      for (int q = lane; q < n * dsupp; q += WAVE) A[q] = Asrc[q];
-      for (int q = lane; q < m * dsupp; q += WAVE) B[q] = Bsrc[q];
-    }
+    for (int q = tid; q < n; q += T) wa[q] = wA[q];
+    for (int q = tid; q < m; q += T) wb[q] = wB[q];
+    if (dA > 0)
+      for (int q = tid; q < n * dA; q += T) A[q] = Asrc[q];
+    if (dB > 0)
+      for (int q = tid; q < m * dB; q += T) B[q] = Bsrc[q];
     int mode = a.cost_mode;
     if (mode == LP_COST_SPARSE) {
       if (idsrc) {
-        for (int q = lane; q < m; q += WAVE) ids[q] = idsrc[q];
+        for (int q = tid; q < m; q += T) ids[q] = idsrc[q];
       } else {
         mode = LP_COST_HIST;  // two centroids: both are full V-bin histograms
       }
     }
     __syncthreads();
-    Cost C{mode, dsupp, n, a.V, A, B, ids, a.dist,
-           a.gcost ? a.gcost + (size_t)job * a.gstride : nullptr};
+    lp::Problem P;
+    P.n = n;
+    P.m = m;
+    P.wa = wa;
+    P.wb = wb;
+    P.C.mode = mode;
+    P.C.d = a.d;
+    P.C.n = n;
+    P.C.V = a.V;
+    P.C.A = dA > 0 ? A : Asrc;
+    P.C.B = dB > 0 ? B : Bsrc;
+    P.C.ids = ids;
+    P.C.dist = a.dist;
+    P.C.g = a.gcost ? a.gcost + (size_t)job * a.gstride : nullptr;
+    P.table = a.table;
+    P.gt = nullptr;
+    P.tab_rel = 0x1p-22f;
+    P.cpt = a.cpt;
     int piv = 0;
-    const double v = solve_lp(C, n, m, wa, wb, lds, L, &piv, a.status);
-    if (lane == 0) {
+    const double v = lp::solve(P, lds, L, &piv, a.status);
+    if (tid == 0) {
       a.out[job] += v;
       if (a.pivots) a.pivots[job] += piv;
     }
@@ -345,21 +102,61 @@ __global__ void k_zero_int(int *p, int n) {
 
 }  // namespace
 
-size_t d2mi_lp_lds_bytes(int n_max, int m_max, int dsupp) {
-  return LpLds(n_max, m_max, dsupp > 0 ? dsupp : 1).total;
+static int env_int(const char *name, int dflt) {
+  const char *v = getenv(name);
+  return v && *v ? atoi(v) : dflt;
 }
 
-int d2mi_launch_transport(d2mi_ctx *ctx, const LpArgs &a) {
-  if (a.njobs <= 0) return D2MI_OK;
-  const int dsupp = a.cost_mode == LP_COST_EUCLID ? a.d : 0;
-  const size_t lds = d2mi_lp_lds_bytes(a.n_max, a.m_max, dsupp);
-  if (lds > ctx->lds_max) {
-    d2mi_set_error("transport LP of %d x %d (d=%d) needs %zu B of LDS (limit %zu)", a.n_max,
-                   a.m_max, dsupp, lds, ctx->lds_max);
+int d2mi_plan_transport(d2mi_ctx *ctx, LpArgs &a) {
+  const bool euclid = a.cost_mode == LP_COST_EUCLID;
+  const bool sparse = a.cost_mode == LP_COST_SPARSE;
+  const size_t nn = (size_t)a.n_max + a.m_max;
+  if (nn > 32767 || (size_t)a.n_max * a.m_max >= (1u << 24)) {
+    d2mi_set_error("transport LP of %d x %d is beyond the solver's size limit", a.n_max, a.m_max);
     return D2MI_EUNSUPPORTED;
   }
-  const int grid = std::min(a.njobs, ctx->num_cu * 16);
-  k_transport<<<grid, WAVE, lds, ctx->stream>>>(a);
+  // what goes to LDS, in order of value: the FP32 cost table, then the
+  // row supports, then the column supports
+  const size_t cap = ctx->lds_max;
+  auto bytes = [&](bool tab, bool sa, bool sb) {
+    return lp::Layout(a.n_max, a.m_max, euclid && sa ? a.d : 0, euclid && sb ? a.d : 0, tab, sparse)
+        .total;
+  };
+  bool tab = bytes(true, false, false) <= cap && env_int("D2MI_LP_TABLE", 1) != 0;
+  bool sa = euclid && bytes(tab, true, false) <= cap;
+  bool sb = euclid && bytes(tab, sa, true) <= cap && (size_t)a.m_max * a.d <= 4096;
+  if (bytes(tab, sa, sb) > cap) {
+    d2mi_set_error("transport LP of %d x %d needs %zu B of LDS (limit %zu)", a.n_max, a.m_max,
+                   bytes(tab, sa, sb), cap);
+    return D2MI_EUNSUPPORTED;
+  }
+  a.table = tab ? lp::TAB_EXACT : lp::TAB_NONE;
+  a.stage_a = sa;
+  a.stage_b = sb;
+  a.lds = bytes(tab, sa, sb);
+  // many small LPs: one wave each; few or large LPs: more waves per LP
+  const size_t cells = (size_t)a.n_max * a.m_max;
+  int T = 64;
+  if (a.njobs < ctx->num_cu * 4) {
+    if (cells >= 8192) T = 256;
+    else if (cells >= 2048) T = 128;
+  }
+  if (a.threads_hint > 0) T = a.threads_hint;
+  T = env_int("D2MI_LP_THREADS", T);
+  T = std::max(64, std::min(1024, T / 64 * 64));
+  a.threads = T;
+  a.cpt = std::max(1, env_int("D2MI_LP_BLOCK", 1024) / T);
+  return D2MI_OK;
+}
+
+int d2mi_launch_transport(d2mi_ctx *ctx, const LpArgs &a0) {
+  if (a0.njobs <= 0) return D2MI_OK;
+  LpArgs a = a0;
+  D2MI_TRY(d2mi_plan_transport(ctx, a));
+  int per_cu = 0;
+  D2MI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_transport, a.threads, a.lds));
+  const int grid = std::min(a.njobs, ctx->num_cu * std::max(1, per_cu));
+  k_transport<<<grid, a.threads, a.lds, a.stream ? a.stream : ctx->stream>>>(a);
   D2MI_HIP(hipGetLastError());
   return D2MI_OK;
 }

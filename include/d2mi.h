@@ -169,6 +169,54 @@ int d2mi_clustering(d2mi_ctx *ctx, int K, int max_iter, int selected_phase, int 
                     int maxIters, double rhoCoeff, int updatePerLoops, int precision,
                     d2mi_cluster_stats *stats);
 
+/* ------------------------------------------------------------- serving */
+
+/* Exact nearest-centroid queries with continuous batching against the
+ * resident centroids of one Euclidean phase: per query, the label and the
+ * distance that the reference's --eval mode computes for one object
+ * (d2_assignment -> d2_labeling, clustering.c:324-376, 528-546; main.cc:253-275):
+ * smallest sqrt(transport-LP value), first minimum on ties.  Queries are
+ * copied at submission; progress() admits waiting queries as streams and HBM
+ * free up (one size class per launch group), launches their work and
+ * collects finished groups without blocking. */
+typedef struct d2mi_server d2mi_server;
+
+typedef struct {
+  int max_batch;     /* queries per launch group (0: 128) */
+  int streams;       /* launch groups in flight at once (0: 4, at most 16) */
+  size_t hbm_bytes;  /* device memory for admitted queries (0: 1 GiB) */
+  int max_m;         /* largest query accepted (0 or above 2048: 2048 support points) */
+  int use_triangle;  /* 1: also prune with W2(c1, c) > 2 W2(x, c1) (Elkan's test) */
+} d2mi_serve_opts;
+
+typedef struct {
+  long long submitted, completed;
+  long long groups;               /* launch groups run */
+  long long lps;                  /* exact transport LPs solved */
+  long long memory_waits;         /* admissions deferred for lack of HBM */
+  long long max_inflight_groups;  /* most groups on the device at once */
+  long long max_group;            /* largest group admitted */
+} d2mi_serve_stats;
+
+/* Opens a server on the centroids resident for `phase` (d2mi_set_centroids);
+ * opts may be NULL.  max_batch = 1 with streams = 1 serves one query per
+ * launch, in arrival order (the plain form). */
+int d2mi_serve_open(d2mi_ctx *ctx, int phase, const d2mi_serve_opts *opts, d2mi_server **out);
+/* Queues one query: m support points (supp: m x d row-major, w: m weights). */
+int d2mi_serve_submit(d2mi_server *srv, int m, const double *w, const double *supp,
+                      long long *ticket);
+/* Collects finished groups and admits waiting queries; *pending (may be
+ * NULL) = submitted minus completed. */
+int d2mi_serve_progress(d2mi_server *srv, int *pending);
+/* 1 and the result when the query is done (the ticket is then released),
+ * 0 while it is pending; latency_ms = completion minus submission. */
+int d2mi_serve_result(d2mi_server *srv, long long ticket, int *label, double *dist,
+                      double *latency_ms);
+/* Blocks until every submitted query is done. */
+int d2mi_serve_drain(d2mi_server *srv);
+int d2mi_serve_stats_get(d2mi_server *srv, d2mi_serve_stats *stats);
+int d2mi_serve_close(d2mi_server *srv);
+
 #ifdef __cplusplus
 }
 #endif
