@@ -39,6 +39,8 @@ namespace lp {
 
 constexpr int WAVE = 64;
 constexpr int MAX_WAVES = 16;
+constexpr int JB = 4;        // nodes per thread per batch in the pointer-jumping rounds
+constexpr int CPT_MAX = 16;  // cells per thread per pricing block (unrolled)
 
 enum CostMode {
   COST_EUCLID = 0,  // squared Euclidean distance of supports (A rows, B columns), FP64
@@ -234,6 +236,8 @@ __device__ double solve(const Problem &P, unsigned char *lds, const Layout &L, i
   Ctl *ctl = (Ctl *)(lds + L.ctl);
   int bank = 0;
   const float inv_n = 1.0f / (float)n;
+  const int Tq = T / n, Tr = T - (T / n) * n;  // a step of T cells = Tq columns + Tr rows
+  const int wid = tid / WAVE;
 
   // ---- cost table and the cost scale for the reduced-cost tolerance
   double cmax = 0.0;
@@ -321,18 +325,34 @@ __device__ double solve(const Problem &P, unsigned char *lds, const Layout &L, i
         short *nxt = lvl + (size_t)(r + 1) * nn;
         const double sg = r == 0 ? -1.0 : 1.0;
         int any = 0;
-        for (int v = tid; v < nn; v += T) {
-          const int a = cur[v];
-          if (a >= 0) {
-            const int aa = cur[a];
-            aD[v] = aS[v] + sg * aS[a];
-            dD[v] = dS[v] + dS[a];
-            nxt[v] = (short)aa;
-            any |= aa >= 0;
-          } else {
-            aD[v] = aS[v];
-            dD[v] = dS[v];
-            nxt[v] = -1;
+        // JB nodes per thread at a time: every load of the batch is issued
+        // before its stores, so the LDS latencies overlap
+        for (int v0 = tid; v0 < nn; v0 += JB * T) {
+          int vv[JB], a[JB], aa[JB], dv[JB], da[JB];
+          double av[JB], aav[JB];
+#pragma unroll
+          for (int u = 0; u < JB; ++u) {
+            vv[u] = v0 + u * T;
+            a[u] = vv[u] < nn ? cur[vv[u]] : -1;
+            av[u] = vv[u] < nn ? aS[vv[u]] : 0.0;
+            dv[u] = vv[u] < nn ? dS[vv[u]] : 0;
+          }
+#pragma unroll
+          for (int u = 0; u < JB; ++u) {
+            const int x = a[u] >= 0 ? a[u] : 0;
+            aa[u] = a[u] >= 0 ? cur[x] : -1;
+            aav[u] = aS[x];
+            da[u] = dS[x];
+          }
+#pragma unroll
+          for (int u = 0; u < JB; ++u) {
+            if (vv[u] < nn) {
+              const bool up = a[u] >= 0;
+              aD[vv[u]] = up ? av[u] + sg * aav[u] : av[u];
+              dD[vv[u]] = up ? dv[u] + da[u] : dv[u];
+              nxt[vv[u]] = (short)aa[u];
+              any |= aa[u] >= 0;
+            }
           }
         }
         any = block_or(any, ctl, bank, nw);
@@ -354,30 +374,51 @@ __device__ double solve(const Problem &P, unsigned char *lds, const Layout &L, i
       if (base >= N) base -= N;
       double best = -eps;
       int bestc = INT_MAX;
-      for (int i = 0; i < P.cpt; ++i) {
-        const int off = tid + i * T;
-        if (scanned + off >= N) break;
-        int e = base + off;
-        if (e >= N) e -= N;
-        int k, j;
-        split_cell(e, n, inv_n, k, j);
-        const double pu = pot[k], pv = pot[n + j];
-        double r;
-        if (!table) {
-          r = C(k, j) - pu - pv;
-        } else {
-          const double c = (double)ct[e];
-          r = c - pu - pv;
-          if (exact_mode) {
-            if (r < (double)P.tab_rel * fabs(c) + 1e-37 + eps) r = C(k, j) - pu - pv;
-            else continue;
+      // this thread's cells base+tid, base+tid+T, ...: (k, j) stepped without divides
+      int e = base + tid;
+      if (e >= N) e -= N;
+      int k, j;
+      split_cell(e, n, inv_n, k, j);
+      if (table && !exact_mode) {
+#pragma unroll
+        for (int i = 0; i < CPT_MAX; ++i) {
+          if (i < P.cpt && scanned + tid + i * T < N) {
+            const double r = (double)ct[e] - pot[k] - pot[n + j];
+            if (bland) {
+              if (r < -eps && e < bestc) { bestc = e; best = r; }
+            } else if (r < best || (r == best && bestc != INT_MAX && e < bestc)) {
+              best = r;
+              bestc = e;
+            }
           }
+          e += T; k += Tr; j += Tq;
+          if (k >= n) { k -= n; ++j; }
+          if (e >= N) { e -= N; j -= m; }
         }
-        if (bland) {
-          if (r < -eps && e < bestc) { bestc = e; best = r; }
-        } else if (r < best || (r == best && bestc != INT_MAX && e < bestc)) {
-          best = r;
-          bestc = e;
+      } else {
+        for (int i = 0; i < P.cpt && scanned + tid + i * T < N; ++i) {
+          const double pu = pot[k], pv = pot[n + j];
+          double r = 0.0;
+          bool priced = true;
+          if (!table) {
+            r = C(k, j) - pu - pv;
+          } else {
+            const double c = (double)ct[e];
+            r = c - pu - pv;
+            if (r < (double)P.tab_rel * fabs(c) + 1e-37 + eps) r = C(k, j) - pu - pv;
+            else priced = false;
+          }
+          if (priced) {
+            if (bland) {
+              if (r < -eps && e < bestc) { bestc = e; best = r; }
+            } else if (r < best || (r == best && bestc != INT_MAX && e < bestc)) {
+              best = r;
+              bestc = e;
+            }
+          }
+          e += T; k += Tr; j += Tq;
+          if (k >= n) { k -= n; ++j; }
+          if (e >= N) { e -= N; j -= m; }
         }
       }
       if (bland) {
@@ -418,70 +459,128 @@ __device__ double solve(const Problem &P, unsigned char *lds, const Layout &L, i
         if (k & 1) v = lvl[(size_t)bit * nn + v];
       return v;
     };
-    int Dstar = 0;
-    for (int top = min(da, db); top >= 0; top -= WAVE) {
-      const int D = top - lane;
-      const bool eq = D >= 0 && anc(a, da - D) == anc(b, db - D);
-      const unsigned long long bal = __ballot(eq);
-      if (bal) {
-        Dstar = top - (__ffsll((long long)bal) - 1);
-        break;
+    // ---- pivot.  Fast path: both arms of the cycle lie within one window of
+    // 64 depths below max(da, db), so lane l of wave 0 holds the depth-(top-l)
+    // node of each arm and does the ratio test, the flow update and the
+    // re-hang with shuffles; other waves go straight to the barrier.
+    const int top = max(da, db);
+    const int Dl = top - lane;
+    const int xa = (Dl >= 0 && Dl <= da) ? anc(a, da - Dl) : -1;
+    const int xb = (Dl >= 0 && Dl <= db) ? anc(b, db - Dl) : -2;
+    const unsigned long long bal = __ballot(Dl >= 0 && Dl <= min(da, db) && xa == xb);
+    if (bal != 0) {
+      if (wid == 0) {
+        const int Dstar = top - (__ffsll((long long)bal) - 1);
+        const int nA = da - Dstar;
+        const int tA = da - Dl, tB = db - Dl;
+        const bool vA = Dl > Dstar && Dl <= da, vB = Dl > Dstar && Dl <= db;
+        const double fA = vA ? flow[xa] : 0.0, fB = vB ? flow[xb] : 0.0;
+        const int cA = vA ? edge_cell(xa, par[xa]) : INT_MAX;
+        const int cB = vB ? edge_cell(xb, par[xb]) : INT_MAX;
+        double th = DBL_MAX;
+        int key = INT_MAX;
+        if (vA && (tA & 1) == 0) { th = fA; key = bland ? cA : tA; }
+        if (vB && (tB & 1) == 0) {
+          const int kb = bland ? cB : nA + tB;
+          if (fB < th || (fB == th && kb < key)) { th = fB; key = kb; }
+        }
+        const double theta = wave_min_d(th);
+        const int keyw = wave_min_i(th == theta ? key : INT_MAX);
+        int qpos = keyw;
+        if (bland) {
+          int cand = INT_MAX;
+          if (vA && (tA & 1) == 0 && fA == theta && cA == keyw) cand = tA;
+          if (vB && (tB & 1) == 0 && fB == theta && cB == keyw) cand = min(cand, nA + tB);
+          qpos = wave_min_i(cand);
+        }
+        const int qside = qpos >= nA ? 1 : 0;
+        const int qt = qside ? qpos - nA : qpos;
+        const int src = lane > 0 ? lane - 1 : 0;  // one step down the arm
+        const int xa_p = __shfl(xa, src, WAVE), xb_p = __shfl(xb, src, WAVE);
+        const double fA_p = __shfl(fA, src, WAVE), fB_p = __shfl(fB, src, WAVE);
+        if (vA) {
+          if (qside == 0 && tA <= qt) {
+            par[xa] = tA == 0 ? b : xa_p;
+            flow[xa] = tA == 0 ? theta : fA_p + (((tA - 1) & 1) ? theta : -theta);
+          } else {
+            flow[xa] = fA + ((tA & 1) ? theta : -theta);
+          }
+        }
+        if (vB) {
+          if (qside == 1 && tB <= qt) {
+            par[xb] = tB == 0 ? a : xb_p;
+            flow[xb] = tB == 0 ? theta : fB_p + (((tB - 1) & 1) ? theta : -theta);
+          } else {
+            flow[xb] = fB + ((tB & 1) ? theta : -theta);
+          }
+        }
       }
-    }
-    const int nA = da - Dstar, nB = db - Dstar, np = nA + nB;
+    } else {
+      int Dstar = 0;
+      for (int top = min(da, db); top >= 0; top -= WAVE) {
+        const int D = top - lane;
+        const bool eq = D >= 0 && anc(a, da - D) == anc(b, db - D);
+        const unsigned long long bal = __ballot(eq);
+        if (bal) {
+          Dstar = top - (__ffsll((long long)bal) - 1);
+          break;
+        }
+      }
+      const int nA = da - Dstar, nB = db - Dstar, np = nA + nB;
 
-    // ---- path nodes and the ratio test over the minus edges (even t)
-    double th = DBL_MAX;
-    int thk = INT_MAX;
-    for (int q = tid; q < np; q += T) {
-      const bool sideB = q >= nA;
-      const int t = sideB ? q - nA : q;
-      const int v = anc(sideB ? b : a, t);
-      path[q] = v;
-      const double f = flow[v];
-      pfl[q] = f;
-      if ((t & 1) == 0) {
-        const int key = bland ? edge_cell(v, par[v]) : q;
-        if (f < th || (f == th && key < thk)) { th = f; thk = key; }
-      }
-    }
-    block_argmin(th, thk, ctl, bank, nw);
-    // the leaving position: recover it from the key
-    int qpos = thk;
-    if (bland) {
-      // the key is a cell index: find the position holding it
-      int found = INT_MAX;
+      // ---- path nodes and the ratio test over the minus edges (even t)
+      double th = DBL_MAX;
+      int thk = INT_MAX;
       for (int q = tid; q < np; q += T) {
         const bool sideB = q >= nA;
         const int t = sideB ? q - nA : q;
+        const int v = anc(sideB ? b : a, t);
+        path[q] = v;
+        const double f = flow[v];
+        pfl[q] = f;
         if ((t & 1) == 0) {
-          const int v = path[q];
-          if (pfl[q] == th && edge_cell(v, par[v]) == thk) found = min(found, q);
+          const int key = bland ? edge_cell(v, par[v]) : q;
+          if (f < th || (f == th && key < thk)) { th = f; thk = key; }
         }
       }
-      double dummy = 0.0;
-      block_argmin(dummy, found, ctl, bank, nw);
-      qpos = found;
-    }
-    const double theta = th;
-    const int qside = qpos >= nA ? 1 : 0;
-    const int qt = qside ? qpos - nA : qpos;
-
-    // ---- flows around the cycle and the re-hang of the cut subtree
-    for (int q = tid; q < np; q += T) {
-      const int side = q >= nA ? 1 : 0;
-      const int t = side ? q - nA : q;
-      const int v = path[q];
-      if (side == qside && t <= qt) {
-        if (t == 0) {
-          par[v] = side == 0 ? b : a;
-          flow[v] = theta;
-        } else {
-          par[v] = path[q - 1];
-          flow[v] = pfl[q - 1] + (((t - 1) & 1) ? theta : -theta);
+      block_argmin(th, thk, ctl, bank, nw);
+      // the leaving position: recover it from the key
+      int qpos = thk;
+      if (bland) {
+        // the key is a cell index: find the position holding it
+        int found = INT_MAX;
+        for (int q = tid; q < np; q += T) {
+          const bool sideB = q >= nA;
+          const int t = sideB ? q - nA : q;
+          if ((t & 1) == 0) {
+            const int v = path[q];
+            if (pfl[q] == th && edge_cell(v, par[v]) == thk) found = min(found, q);
+          }
         }
-      } else {
-        flow[v] = pfl[q] + ((t & 1) ? theta : -theta);
+        double dummy = 0.0;
+        block_argmin(dummy, found, ctl, bank, nw);
+        qpos = found;
+      }
+      const double theta = th;
+      const int qside = qpos >= nA ? 1 : 0;
+      const int qt = qside ? qpos - nA : qpos;
+
+      // ---- flows around the cycle and the re-hang of the cut subtree
+      for (int q = tid; q < np; q += T) {
+        const int side = q >= nA ? 1 : 0;
+        const int t = side ? q - nA : q;
+        const int v = path[q];
+        if (side == qside && t <= qt) {
+          if (t == 0) {
+            par[v] = side == 0 ? b : a;
+            flow[v] = theta;
+          } else {
+            par[v] = path[q - 1];
+            flow[v] = pfl[q - 1] + (((t - 1) & 1) ? theta : -theta);
+          }
+        } else {
+          flow[v] = pfl[q] + ((t & 1) ? theta : -theta);
+        }
       }
     }
     __syncthreads();
