@@ -16,8 +16,10 @@
 //     ceil(log2 D)+1 rounds;
 //   * pricing: block search (the next block of T*cpt cells after the last
 //     entering cell; the most negative reduced cost in the first block that
-//     has one); after 50*(n+m)+1000 pivots, Bland's rule (lowest improving
-//     cell, lowest leaving cell on ties), which cannot cycle;
+//     has one, lowest cell on ties); after 50*(n+m)+1000 pivots, Bland's rule
+//     (lowest improving cell, lowest leaving cell on ties), which cannot
+//     cycle.  With the same block size the pivot sequence, and so the value,
+//     does not depend on the workgroup size;
 //   * pivot: every wave finds the apex of the cycle by testing 64 depths at a
 //     time against the lifting table; path nodes, the ratio test, the flow
 //     update and the re-hanging of the cut subtree (parent pointers reversed
@@ -40,7 +42,7 @@ namespace lp {
 constexpr int WAVE = 64;
 constexpr int MAX_WAVES = 16;
 constexpr int JB = 4;        // nodes per thread per batch in the pointer-jumping rounds
-constexpr int CPT_MAX = 16;  // cells per thread per pricing block (unrolled)
+constexpr int CPT_MAX = 32;  // cells per thread per pricing block (unrolled)
 
 enum CostMode {
   COST_EUCLID = 0,  // squared Euclidean distance of supports (A rows, B columns), FP64
@@ -587,13 +589,19 @@ __device__ double solve(const Problem &P, unsigned char *lds, const Layout &L, i
     ++pivots;
   }
 
-  // ---- objective on exact costs over the tree edges
-  double val = 0.0;
+  // ---- objective on exact costs over the tree edges, summed in a fixed
+  // pairwise order so the value does not depend on the workgroup size
+  double *sum = pot == accA ? accB : accA;
   for (int v = tid; v < nn; v += T) {
     const int p = par[v];
-    if (p >= 0) val += flow[v] * C(edge_k(v, p), edge_j(v, p));
+    sum[v] = p >= 0 ? flow[v] * C(edge_k(v, p), edge_j(v, p)) : 0.0;
   }
-  val = block_sum(val, ctl, bank, nw);
+  __syncthreads();
+  for (int stride = 1; stride < nn; stride <<= 1) {
+    for (int i = tid * 2 * stride; i + stride < nn; i += T * 2 * stride) sum[i] += sum[i + stride];
+    __syncthreads();
+  }
+  const double val = sum[0];
   if (pivots_out) *pivots_out = pivots;
   __syncthreads();
   return val;

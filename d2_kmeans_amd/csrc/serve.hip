@@ -5,11 +5,12 @@
 // the centroid with the smallest exact transport distance (first minimum on
 // ties) and that distance, sqrt of the LP value (clustering.c:126-127).
 //
-// Continuous batching.  Queries wait in a host queue.  Whenever one of the
-// server's streams is free and the HBM arena has room, the oldest waiting
-// query's size class (m <= 64, <= 192, larger) is admitted as one launch
-// group: up to max_batch waiting queries of that class, in arrival order,
-// whose inputs, bounds and results fit the arena.  The group's work runs on
+// Continuous batching.  Queries wait in a host queue.  Whenever a stream of
+// a waiting query's size class (m <= 64, <= 192, larger; with three or more
+// streams each class has its own) is free and the HBM arena has room, that
+// class is admitted as one launch group: up to max_batch waiting queries of
+// the class, in arrival order, whose inputs, bounds and results fit the
+// arena.  The group's work runs on
 // its own stream, so groups admitted at different times overlap on the
 // device; finished groups free their stream and memory at the next progress
 // call.  Per group:
@@ -18,11 +19,13 @@
 //      mass from some query point, so
 //          LP >= max( sum_j b_j min_k C_kj , sum_k a_k min_j C_kj )
 //      (exact FP64 costs by direct differences);
-//   2. one exact LP per query for the centroid with the smallest bound;
-//   3. exact LPs for the centroids the bounds cannot rule out: bound above
-//      the first LP value, or (Elkan's test, W2 being a metric)
-//      W2(c1, c) > 2 W2(x, c1) with the centroid-pair distances computed
-//      when the server opens;
+//   2. exact LPs for every centroid whose bound is within a factor tau of
+//      the smallest bound (the nearest centroid is almost always among them);
+//   3. exact LPs for the centroids still unsolved that the bounds cannot
+//      rule out: bound at most the best first-round value, and not excluded
+//      by Elkan's test W2(c1, c) > 2 W2(x, c1) (W2 is a metric; c1 = the
+//      best first-round centroid, centroid-pair distances computed when the
+//      server opens); usually none are left;
 //   4. the smallest exact value wins, lowest centroid index on ties.
 // Exclusions are strict with a relative margin, so a centroid tied with the
 // winner is always solved and the reference's tie rule decides.
@@ -67,8 +70,7 @@ __global__ __launch_bounds__(LB_T) void k_srv_bounds(int K, int s, int d,
                                                      const double *__restrict__ w,
                                                      const double *__restrict__ supp,
                                                      double *__restrict__ lb,
-                                                     double *__restrict__ vals,
-                                                     unsigned char *__restrict__ mask) {
+                                                     double *__restrict__ vals) {
   extern __shared__ __align__(16) double shA[];  // s*d, then 4 waves x LB_KC row minima
   double *rmin_w = shA + (size_t)s * d;
   __shared__ double red[8];
@@ -156,54 +158,68 @@ __global__ __launch_bounds__(LB_T) void k_srv_bounds(int K, int s, int d,
     const size_t o = (size_t)r * K + c;
     lb[o] = fmax(fmax(lc, lr), 0.0);
     vals[o] = 0.0;
-    mask[o] = 0;
   }
 }
 
-// first candidate per query: the smallest bound (lowest index on ties)
-__global__ void k_srv_first(int R, int K, const double *__restrict__ lb, int *c1, double *v1) {
+// first round: centroids whose bound is within tau of the smallest bound
+// (mask = 1; the smallest-bound centroid is always in)
+__global__ void k_srv_round1(int R, int K, double tau, const double *__restrict__ lb,
+                             unsigned char *mask) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= R) return;
-  int best = 0;
-  double bv = lb[(size_t)r * K];
-  for (int c = 1; c < K; ++c) {
-    const double v = lb[(size_t)r * K + c];
-    if (v < bv) { bv = v; best = c; }
+  const double *l = lb + (size_t)r * K;
+  double mn = l[0];
+  for (int c = 1; c < K; ++c) mn = fmin(mn, l[c]);
+  const double cut = mn * tau;
+  for (int c = 0; c < K; ++c) mask[(size_t)r * K + c] = (l[c] <= cut || l[c] == mn) ? 1 : 0;
+}
+
+// best solved centroid so far (lowest index on ties) over mask != 0
+__device__ __forceinline__ void best_solved(const double *v, const unsigned char *mk, int K,
+                                            int &best, double &bv) {
+  best = -1;
+  bv = DBL_MAX;
+  for (int c = 0; c < K; ++c)
+    if (mk[c] && (v[c] < bv || best < 0)) { bv = v[c]; best = c; }
+}
+
+// second round: unsolved centroids the bounds cannot rule out (mask = 2)
+__global__ void k_srv_round2(int R, int K, const double *__restrict__ lb,
+                             const double *__restrict__ vals, const double *__restrict__ cc,
+                             unsigned char *mask, unsigned char *mask2) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= R) return;
+  const size_t o = (size_t)r * K;
+  int f;
+  double u2;
+  best_solved(vals + o, mask + o, K, f, u2);
+  u2 = fmax(u2, 0.0);  // W2^2 to the best first-round centroid
+  const double u = sqrt(u2);
+  for (int c = 0; c < K; ++c) {
+    unsigned char m2 = 0;
+    if (!mask[o + c]) {
+      bool out = lb[o + c] > u2 * (1.0 + EXCL_MARGIN) + 1e-300;
+      if (cc) out = out || cc[(size_t)f * K + c] > 2.0 * u * (1.0 + EXCL_MARGIN);
+      m2 = out ? 0 : 1;
+    }
+    mask2[o + c] = m2;
   }
-  c1[r] = best;
-  v1[r] = 0.0;
 }
 
-// candidates for the second round
-__global__ void k_srv_mask(int R, int K, const double *__restrict__ lb, const int *__restrict__ c1,
-                           const double *__restrict__ v1, const double *__restrict__ cc,
-                           unsigned char *mask) {
-  const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  if (t >= (size_t)R * K) return;
-  const int r = (int)(t / K), c = (int)(t - (size_t)r * K);
-  const int f = c1[r];
-  if (c == f) { mask[t] = 0; return; }
-  const double u2 = fmax(v1[r], 0.0);  // W2^2 to the first candidate
-  bool out = lb[t] > u2 * (1.0 + EXCL_MARGIN) + 1e-300;
-  if (cc) out = out || cc[(size_t)f * K + c] > 2.0 * sqrt(u2) * (1.0 + EXCL_MARGIN);
-  mask[t] = out ? 0 : 1;
-}
-
-__global__ void k_srv_final(int R, int K, const int *__restrict__ c1, const double *__restrict__ v1,
-                            const unsigned char *__restrict__ mask,
+__global__ void k_srv_final(int R, int K, const unsigned char *__restrict__ mask,
+                            const unsigned char *__restrict__ mask2,
                             const double *__restrict__ vals, int *label, double *dist,
                             unsigned long long *nlp) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= R) return;
-  int best = c1[r];
-  double bv = v1[r];
-  int cnt = 1;
+  const size_t o = (size_t)r * K;
+  int best = -1, cnt = 0;
+  double bv = DBL_MAX;
   for (int c = 0; c < K; ++c) {
-    const size_t o = (size_t)r * K + c;
-    if (!mask[o]) continue;
+    if (!mask[o + c] && !mask2[o + c]) continue;
     ++cnt;
-    const double v = vals[o];
-    if (v < bv || (v == bv && c < best)) { bv = v; best = c; }
+    const double v = vals[o + c];
+    if (best < 0 || v < bv) { bv = v; best = c; }
   }
   label[r] = best;
   dist[r] = bv <= 0.0 ? 0.0 : sqrt(bv);
@@ -240,7 +256,7 @@ struct SrvGroup {
   hipEvent_t done = nullptr;
   std::vector<long long> tickets;
   size_t arena_off = 0, arena_bytes = 0;
-  // pinned staging: inputs (p_str, cum, w, supp, ridx) and results
+  // pinned staging: inputs (p_str, cum, w, supp) and results
   void *h_in = nullptr;
   size_t h_in_cap = 0;
   void *h_out = nullptr;
@@ -252,6 +268,7 @@ struct d2mi_server {
   d2mi_ctx *ctx = nullptr;
   int phase = 0, K = 0, s = 0, d = 0;
   d2mi_serve_opts opt{};
+  double opt_tau = 1.1;
   void *arena = nullptr;
   std::map<size_t, size_t> free_blocks;  // offset -> length
   DevMem cc, cs_half;                    // centroid pair W2 [K*K], spare
@@ -296,10 +313,22 @@ void arena_free(d2mi_server *S, size_t off, size_t bytes) {
 
 int size_class(int m) { return m <= 64 ? 0 : (m <= 192 ? 1 : 2); }
 
+// Streams are split by size class when there are at least three: stream 0
+// runs the largest queries, stream 1 the middle class, the rest the small
+// ones, so a query only ever waits behind groups of its own class (the
+// device still runs all of them at once).
+bool slot_serves(const d2mi_server *S, int slot, int cls) {
+  const int n = (int)S->slots.size();
+  if (n < 3) return true;
+  if (slot == 0) return cls == 2;
+  if (slot == 1) return cls == 1;
+  return cls == 0;
+}
+
 // device bytes of a group of R queries with `col` support points in total
 struct GroupLayout {
-  size_t p_str, cum, w, supp, ridx, in_end;                      // inputs (one H2D copy)
-  size_t lb, vals, mask, c1, v1, label, dist, nlp, status, out_end;  // outputs from `label`
+  size_t p_str, cum, w, supp, in_end;                            // inputs (one H2D copy)
+  size_t lb, vals, mask, mask2, label, dist, nlp, status, out_end;  // outputs from `label`
   size_t total;
   GroupLayout(int R, size_t col, int d, int K) {
     size_t o = 0;
@@ -307,13 +336,11 @@ struct GroupLayout {
     cum = o; o = a256(o + sizeof(long long) * R);
     w = o; o = a256(o + sizeof(double) * col);
     supp = o; o = a256(o + sizeof(double) * col * d);
-    ridx = o; o = a256(o + sizeof(int) * R);
     in_end = o;
     lb = o; o = a256(o + sizeof(double) * (size_t)R * K);
     vals = o; o = a256(o + sizeof(double) * (size_t)R * K);
     mask = o; o = a256(o + (size_t)R * K);
-    c1 = o; o = a256(o + sizeof(int) * R);
-    v1 = o; o = a256(o + sizeof(double) * R);
+    mask2 = o; o = a256(o + (size_t)R * K);
     label = o; o = a256(o + sizeof(int) * R);
     dist = o; o = a256(o + sizeof(double) * R);
     nlp = o; o = a256(o + sizeof(unsigned long long));
@@ -325,7 +352,7 @@ struct GroupLayout {
 
 size_t query_bytes(int m, int d, int K) {
   // share of one query in a group (bounds, values, mask, per-query scalars)
-  return sizeof(double) * (size_t)m * (d + 1) + (sizeof(double) * 2 + 1) * (size_t)K + 64;
+  return sizeof(double) * (size_t)m * (d + 1) + (sizeof(double) * 2 + 2) * (size_t)K + 64;
 }
 
 int launch_lp_round(d2mi_server *S, SrvGroup &G, const GroupLayout &L, char *base, int round) {
@@ -352,17 +379,10 @@ int launch_lp_round(d2mi_server *S, SrvGroup &G, const GroupLayout &L, char *bas
   a.stream = G.stream;
   const size_t cells = (size_t)S->s * m_max;
   a.threads_hint = cells >= 8192 ? 256 : (cells >= 2048 ? 128 : 64);
-  if (round == 1) {
-    a.njobs = R;
-    a.job_a = (const int *)(base + L.c1);
-    a.job_b = (const int *)(base + L.ridx);
-    a.out = (double *)(base + L.v1);
-  } else {
-    a.njobs = R * S->K;
-    a.implicit_K = S->K;
-    a.mask = (const unsigned char *)(base + L.mask);
-    a.out = (double *)(base + L.vals);
-  }
+  a.njobs = R * S->K;
+  a.implicit_K = S->K;
+  a.mask = (const unsigned char *)(base + (round == 1 ? L.mask : L.mask2));
+  a.out = (double *)(base + L.vals);
   return d2mi_launch_transport(ctx, a);
 }
 
@@ -383,13 +403,11 @@ int launch_group(d2mi_server *S, SrvGroup &G) {
   int *hp = (int *)(h + L.p_str);
   long long *hc = (long long *)(h + L.cum);
   double *hw = (double *)(h + L.w), *hs = (double *)(h + L.supp);
-  int *hr = (int *)(h + L.ridx);
   long long o = 0;
   for (int r = 0; r < R; ++r) {
     const SrvQuery &Q = S->q[G.tickets[r]];
     hp[r] = Q.m;
     hc[r] = o;
-    hr[r] = r;
     memcpy(hw + o, Q.w.data(), sizeof(double) * Q.m);
     memcpy(hs + (size_t)o * d, Q.supp.data(), sizeof(double) * Q.m * d);
     o += Q.m;
@@ -402,22 +420,21 @@ int launch_group(d2mi_server *S, SrvGroup &G) {
   k_srv_bounds<<<R * K, LB_T, lds, st>>>(
       K, s, d, P.cs64.as<double>(), P.cw64.as<double>(), (const int *)(base + L.p_str),
       (const long long *)(base + L.cum), (const double *)(base + L.w),
-      (const double *)(base + L.supp), (double *)(base + L.lb), (double *)(base + L.vals),
-      (unsigned char *)(base + L.mask));
-  k_srv_first<<<(R + 63) / 64, 64, 0, st>>>(R, K, (const double *)(base + L.lb),
-                                            (int *)(base + L.c1), (double *)(base + L.v1));
+      (const double *)(base + L.supp), (double *)(base + L.lb), (double *)(base + L.vals));
+  k_srv_round1<<<(R + 63) / 64, 64, 0, st>>>(R, K, S->opt_tau, (const double *)(base + L.lb),
+                                             (unsigned char *)(base + L.mask));
   D2MI_HIP(hipGetLastError());
   D2MI_TRY(launch_lp_round(S, G, L, base, 1));
-  k_srv_mask<<<(unsigned)(((size_t)R * K + 255) / 256), 256, 0, st>>>(
-      R, K, (const double *)(base + L.lb), (const int *)(base + L.c1),
-      (const double *)(base + L.v1), S->opt.use_triangle ? S->cc.as<double>() : nullptr,
-      (unsigned char *)(base + L.mask));
+  k_srv_round2<<<(R + 63) / 64, 64, 0, st>>>(
+      R, K, (const double *)(base + L.lb), (const double *)(base + L.vals),
+      S->opt.use_triangle ? S->cc.as<double>() : nullptr, (unsigned char *)(base + L.mask),
+      (unsigned char *)(base + L.mask2));
   D2MI_HIP(hipGetLastError());
   D2MI_TRY(launch_lp_round(S, G, L, base, 2));
   k_srv_final<<<(R + 63) / 64, 64, 0, st>>>(
-      R, K, (const int *)(base + L.c1), (const double *)(base + L.v1),
-      (const unsigned char *)(base + L.mask), (const double *)(base + L.vals),
-      (int *)(base + L.label), (double *)(base + L.dist), (unsigned long long *)(base + L.nlp));
+      R, K, (const unsigned char *)(base + L.mask), (const unsigned char *)(base + L.mask2),
+      (const double *)(base + L.vals), (int *)(base + L.label), (double *)(base + L.dist),
+      (unsigned long long *)(base + L.nlp));
   D2MI_HIP(hipGetLastError());
   D2MI_HIP(hipMemcpyAsync(G.h_out, base + L.label, L.out_end - L.label, hipMemcpyDeviceToHost, st));
   D2MI_HIP(hipEventRecord(G.done, st));
@@ -462,13 +479,16 @@ int retire(d2mi_server *S, SrvGroup &G) {
 int admit(d2mi_server *S) {
   for (;;) {
     if (S->queue.empty()) return D2MI_OK;
-    int slot = -1;
-    for (size_t i = 0; i < S->slots.size(); ++i)
-      if (!S->slots[i].busy) { slot = (int)i; break; }
+    // the class of the oldest waiting query that has a free stream of its own
+    int slot = -1, cls = -1;
+    for (long long t : S->queue) {
+      const int c = S->q[t].cls;
+      for (size_t i = 0; i < S->slots.size(); ++i)
+        if (!S->slots[i].busy && slot_serves(S, (int)i, c)) { slot = (int)i; break; }
+      if (slot >= 0) { cls = c; break; }
+    }
     if (slot < 0) return D2MI_OK;
     SrvGroup &G = S->slots[slot];
-    // the oldest query's class; up to max_batch queries of that class in order
-    const int cls = S->q[S->queue.front()].cls;
     std::vector<long long> pick;
     size_t col = 0;
     for (long long t : S->queue) {

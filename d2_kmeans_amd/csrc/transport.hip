@@ -145,7 +145,7 @@ int d2mi_plan_transport(d2mi_ctx *ctx, LpArgs &a) {
   T = env_int("D2MI_LP_THREADS", T);
   T = std::max(64, std::min(1024, T / 64 * 64));
   a.threads = T;
-  a.cpt = std::max(1, env_int("D2MI_LP_BLOCK", 1024) / T);
+  a.cpt = std::max(1, std::min(lp::CPT_MAX, env_int("D2MI_LP_BLOCK", 2048) / T));
   return D2MI_OK;
 }
 

@@ -81,8 +81,10 @@ def test_alone_and_in_the_mix_agree(setup):
     mix_l, mix_d, lat, _ = mix.replay(queries_of(qph), arrivals)
     st = mix.stats()
     mix.close()
+    # the LP's pivot sequence and final sum do not depend on the group it
+    # runs in: the same query gives the same bits alone and in the mix
     np.testing.assert_array_equal(np.array(alone_l), mix_l)
-    np.testing.assert_allclose(np.array(alone_d), mix_d, rtol=1e-12, atol=1e-14)
+    np.testing.assert_array_equal(np.array(alone_d), mix_d)
     np.testing.assert_array_equal(mix_l, lab)
     assert (lat > 0).all()
     say("mix stats", st)
