@@ -31,7 +31,10 @@ constexpr int NT = 256;        // threads per workgroup (4 waves of 64)
 constexpr int MAXD_FUSED = 8;  // Euclidean cost fused on the fly for d <= 8
 constexpr int FLAG_BADLABEL = 1, FLAG_EMPTY = 2, FLAG_NUMERIC = 4;
 
-enum CostMode { COST_EUCLID = 0, COST_HIST = 1, COST_SPARSE = 2 };
+// COST_TABLE: the ground cost is read from a stored table C[s*cum[i] + k + j*s]
+// (the reference's own C array, clustering_util.c:179), rebuilt after every
+// support update; used for Euclidean d > MAXD_FUSED and for word embeddings.
+enum CostMode { COST_EUCLID = 0, COST_HIST = 1, COST_SPARSE = 2, COST_TABLE = 3 };
 
 template <typename T>
 __device__ __forceinline__ T dexp(T x);
@@ -102,6 +105,8 @@ struct PlanArgs {
   const double *__restrict__ scal;  // [0] = rho, [1] = 1/rho
   T *__restrict__ Z;
   T *__restrict__ Y;
+  const T *__restrict__ ctab;       // COST_TABLE: stored costs (plan layout)
+  T *__restrict__ xout;             // COST_TABLE, support-update passes: X out (plan layout)
   double *__restrict__ rec;         // [(G + K) * R]
   double *__restrict__ resid_part;  // [G * 3]
 };
@@ -110,8 +115,11 @@ struct PlanArgs {
 // the Euclidean dimension at compile time (straight-line code)
 template <typename T, int CM, int DIM = 0>
 __device__ __forceinline__ T cost_raw(int k, int j, int s, int d, int V, const T *cs,
-                                      const T *xs, const int *ids, const T *__restrict__ dist) {
-  if constexpr (CM == COST_EUCLID) {
+                                      const T *xs, const int *ids, const T *__restrict__ dist,
+                                      const T *__restrict__ tab = nullptr) {
+  if constexpr (CM == COST_TABLE) {
+    return tab[k + (size_t)j * s];
+  } else if constexpr (CM == COST_EUCLID) {
     T acc = 0;
     if constexpr (DIM > 0) {
 #pragma unroll
@@ -260,8 +268,9 @@ void k_badmm_plan(PlanArgs<T> a, const int *__restrict__ g_perm, const int *__re
         for (int r = 0; r < RMAX; ++r) {
           const int e = tid + r * NT;
           if (e < E) {
-            const T c = scale_cost<T>(cost_raw<T, CM, DIM>(k, j, s, d, a.V, cs, xs, ids, a.dist),
-                                      rho, inv_rho);
+            const T c = scale_cost<T>(
+                cost_raw<T, CM, DIM>(k, j, s, d, a.V, cs, xs, ids, a.dist, a.ctab + base), rho,
+                inv_rho);
             const int li = k + j * sp;
             const T z = reinit ? wl[j] * cwl[k] : zr[r];
             A[li] = z * dexp<T>(-(c + yr[r])) + (T)1e-9;
@@ -273,8 +282,9 @@ void k_badmm_plan(PlanArgs<T> a, const int *__restrict__ g_perm, const int *__re
         }
       } else {
         for (int e = tid; e < E; e += NT) {
-          const T c = scale_cost<T>(cost_raw<T, CM, DIM>(k, j, s, d, a.V, cs, xs, ids, a.dist),
-                                    rho, inv_rho);
+          const T c = scale_cost<T>(
+              cost_raw<T, CM, DIM>(k, j, s, d, a.V, cs, xs, ids, a.dist, a.ctab + base), rho,
+              inv_rho);
           const T z = reinit ? wl[j] * cwl[k] : a.Z[base + e];
           const T y = a.first ? (T)0 : a.Y[base + e];
           const int li = k + j * sp;
@@ -391,8 +401,9 @@ void k_badmm_plan(PlanArgs<T> a, const int *__restrict__ g_perm, const int *__re
         const int li = k + j * sp;
         const T z = F64 ? (A[li] / g[k]) * cwl[k] : A[li] * g[k];
         if (a.resid) {  // step 6 (:300-305), and the exit objective
-          const T c = scale_cost<T>(cost_raw<T, CM, DIM>(k, j, s, d, a.V, cs, xs, ids, a.dist),
-                                    rho, inv_rho);
+          const T c = scale_cost<T>(
+              cost_raw<T, CM, DIM>(k, j, s, d, a.V, cs, xs, ids, a.dist, a.ctab + base), rho,
+              inv_rho);
           const T x = Xs[li];
           const T z0 = reinit ? wl[j] * cwl[k] : a.Z[base + e];
           r_obj += (double)c * (double)x;
@@ -401,6 +412,7 @@ void k_badmm_plan(PlanArgs<T> a, const int *__restrict__ g_perm, const int *__re
         }
         a.Z[base + e] = z;
         a.Y[base + e] = a.keepx ? B[li] + (Xs[li] - z) : B[li] - z;
+        if (CM == COST_TABLE && a.upd) a.xout[base + e] = Xs[li];  // for the support GEMM
         k += dk;
         j += dj;
         if (k >= s) { k -= s; ++j; }
@@ -436,6 +448,7 @@ struct FinArgs {
   double *__restrict__ cs64;
   T *__restrict__ csT;
   double *__restrict__ resid_out;  // [iters][3]
+  double *__restrict__ rxout;      // COST_TABLE support passes: X row sums per label [K*s]
   int *__restrict__ flags;
 };
 
@@ -465,6 +478,8 @@ __global__ __launch_bounds__(NT) void k_badmm_finalize(FinArgs<T> a) {
       a.cwT[(size_t)l * s + k] = (T)v;
     }
   }
+  if (a.upd && a.rxout)
+    for (int k = tid; k < s; k += NT) a.rxout[(size_t)l * s + k] = sums[s + s * d + k];
   if (a.upd) {
     for (int q = tid; q < s * d; q += NT) {
       const int k = q / d;
@@ -488,6 +503,169 @@ __global__ __launch_bounds__(NT) void k_badmm_finalize(FinArgs<T> a) {
   }
 }
 
+// ------------------------------------------------------------ stored costs
+
+template <typename T>
+struct TabArgs {
+  int N, s, d, chunk;
+  const int *__restrict__ perm;  // label-sorted order
+  const int *__restrict__ p_str;
+  const long long *__restrict__ cum;
+  const int *__restrict__ label;
+  const T *__restrict__ supp;    // Euclidean supports [col*d]
+  const int *__restrict__ sym;   // word ids [col]
+  const T *__restrict__ vocab;   // word embeddings [V*d]
+  const T *__restrict__ csupp;   // centroid supports [K*s*d]
+  T *__restrict__ ctab;          // out, plan layout
+};
+
+constexpr int TAB_KC = 16;  // centroid rows per thread
+
+// calculate_distmat (centroid_util.c:8-36): _dpdist2 for Euclidean phases,
+// _dpdist2_sym (blas_like64.c:210-220) for word embeddings, by direct
+// differences.  A workgroup walks a chunk of the label-sorted order with its
+// label's centroid supports in LDS (dimension-major, so a thread's 16 rows
+// are contiguous); thread <-> (column j, chunk of 16 rows).
+template <typename T, bool WORD>
+__global__ __launch_bounds__(NT) void k_badmm_cost_table(TabArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  T *cs = (T *)smem;  // [d][s] of the current label
+  const int s = a.s, d = a.d, tid = threadIdx.x;
+  const int nkc = (s + TAB_KC - 1) / TAB_KC;
+  const int p0 = blockIdx.x * a.chunk, p1 = min(a.N, p0 + a.chunk);
+  int cur = -1;
+  for (int p = p0; p < p1; ++p) {
+    const int o = a.perm[p], l = a.label[o], m = a.p_str[o];
+    const long long off = a.cum[o];
+    if (l != cur) {
+      __syncthreads();
+      for (int q = tid; q < s * d; q += NT) {
+        const int k = q / d, t = q - k * d;
+        cs[t * s + k] = a.csupp[(size_t)l * s * d + q];
+      }
+      __syncthreads();
+      cur = l;
+    }
+    T *C = a.ctab + (size_t)s * off;
+    for (int u = tid; u < m * nkc; u += NT) {
+      const int j = u / nkc, kc = (u - j * nkc) * TAB_KC;
+      const T *x = WORD ? a.vocab + (size_t)a.sym[off + j] * d : a.supp + (size_t)(off + j) * d;
+      T acc[TAB_KC];
+#pragma unroll
+      for (int r = 0; r < TAB_KC; ++r) acc[r] = 0;
+      for (int t = 0; t < d; ++t) {
+        const T xv = x[t];
+        const T *crow = cs + t * s + kc;
+#pragma unroll
+        for (int r = 0; r < TAB_KC; ++r) {
+          if (kc + r < s) {
+            const T df = crow[r] - xv;
+            acc[r] += df * df;
+          }
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < TAB_KC; ++r)
+        if (kc + r < s) C[kc + r + (size_t)j * s] = acc[r];
+    }
+  }
+}
+
+template <typename T>
+struct SuppArgs {
+  int N, s, d, chunk;
+  const int *__restrict__ perm;
+  const int *__restrict__ p_str;
+  const long long *__restrict__ cum;
+  const int *__restrict__ label;
+  const T *__restrict__ supp;
+  const int *__restrict__ sym;
+  const T *__restrict__ vocab;
+  const T *__restrict__ X;  // plan layout, from the support-update pass
+  double *__restrict__ rec;  // [(G + K) * s * d]
+};
+
+constexpr int SUP_KPT = 16;  // rows per thread: s <= 4 * SUP_KPT
+
+// Step 5 numerators: sum over the label's objects of supp_i X_i^T
+// (centroid_Bregman.c:199-212, one dgemm per object) or of X_i[k, j] *
+// vocab[id_j] (:229-247).  Records per (workgroup, label) at g + l as in the
+// plan kernel; a thread owns dimension t = t0 + lane and rows k = wave + 4 i,
+// with FP64 accumulators.
+template <typename T, bool WORD>
+__global__ __launch_bounds__(NT) void k_badmm_supp_num(SuppArgs<T> a) {
+  const int s = a.s, d = a.d, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int g = blockIdx.x;
+  const int p0 = g * a.chunk, p1 = min(a.N, p0 + a.chunk);
+  const size_t R = (size_t)s * d;
+  for (int t0 = 0; t0 < d; t0 += 64) {
+    const int t = t0 + lane;
+    double acc[SUP_KPT];
+    int cur = -1;
+    for (int p = p0; p < p1; ++p) {
+      const int o = a.perm[p], l = a.label[o], m = a.p_str[o];
+      const long long off = a.cum[o];
+      if (l != cur) {
+        if (cur >= 0) {
+#pragma unroll
+          for (int i = 0; i < SUP_KPT; ++i) {
+            const int k = wv + 4 * i;
+            if (k < s && t < d) a.rec[(size_t)(g + cur) * R + (size_t)k * d + t] = acc[i];
+          }
+        }
+#pragma unroll
+        for (int i = 0; i < SUP_KPT; ++i) acc[i] = 0.0;
+        cur = l;
+      }
+      const T *Xo = a.X + (size_t)s * off;
+      for (int j = 0; j < m; ++j) {
+        const T *x = WORD ? a.vocab + (size_t)a.sym[off + j] * d : a.supp + (size_t)(off + j) * d;
+        const double xv = t < d ? (double)x[t] : 0.0;
+        const T *xc = Xo + (size_t)j * s;
+#pragma unroll
+        for (int i = 0; i < SUP_KPT; ++i) {
+          const int k = wv + 4 * i;
+          if (k < s) acc[i] += (double)xc[k] * xv;
+        }
+      }
+    }
+    if (cur >= 0) {
+#pragma unroll
+      for (int i = 0; i < SUP_KPT; ++i) {
+        const int k = wv + 4 * i;
+        if (k < s && t < d) a.rec[(size_t)(g + cur) * R + (size_t)k * d + t] = acc[i];
+      }
+    }
+  }
+}
+
+// new supports = numerators / X row sums (_dirms, :219-221), records summed
+// in workgroup order
+template <typename T>
+__global__ __launch_bounds__(NT) void k_badmm_supp_fin(int s, int d, int chunk,
+                                                       const int *__restrict__ seg,
+                                                       const double *__restrict__ rec,
+                                                       const double *__restrict__ rx,
+                                                       double *__restrict__ cs64,
+                                                       T *__restrict__ csT, int *flags) {
+  const int l = blockIdx.x;
+  const int lo = seg[l], hi = seg[l + 1];
+  const int glo = lo / chunk, ghi = (hi - 1) / chunk;
+  const size_t R = (size_t)s * d;
+  for (size_t q = threadIdx.x; q < R; q += NT) {
+    double v = 0.0;
+    for (int g = glo; g <= ghi; ++g) v += rec[(size_t)(g + l) * R + q];
+    const double r = rx[(size_t)l * s + q / d];
+    if (!(r > 0.0)) {
+      atomicOr(flags, FLAG_NUMERIC);
+      continue;
+    }
+    const double c = v / r;
+    cs64[(size_t)l * R + q] = c;
+    csT[(size_t)l * R + q] = (T)c;
+  }
+}
+
 // ---------------------------------------------------------------- setup
 
 template <typename T>
@@ -506,6 +684,7 @@ struct SetupArgs {
   const T *__restrict__ csupp;
   T *__restrict__ Z;
   T *__restrict__ Y;
+  const T *__restrict__ ctab;
   double *__restrict__ part;  // [gridDim.x]
 };
 
@@ -536,7 +715,7 @@ __global__ __launch_bounds__(NT) void k_badmm_setup(SetupArgs<T> a) {
     const bool init = a.force_init || a.lsw[o] == 1;
     for (int e = tid; e < s * m; e += NT) {
       const int j = e / s, k = e - j * s;
-      const T c = cost_raw<T, CM>(k, j, s, d, a.V, cs, xs, ids, a.dist);
+      const T c = cost_raw<T, CM>(k, j, s, d, a.V, cs, xs, ids, a.dist, a.ctab + base);
       csum += fabs((double)c);
       if (a.write_state) {  // the engine does this inside its first pass instead
         if (init) a.Z[base + e] = a.w[off + j] * a.cw[(size_t)l * s + k];
@@ -655,6 +834,8 @@ struct PlainArgs {
   T *__restrict__ Y;
   T *__restrict__ X;       // [s*col] scratch
   double *__restrict__ acc;  // [K * s * (d + 2)]: weights, supports, X row sums
+  const T *__restrict__ ctab;   // COST_TABLE costs
+  const T *__restrict__ vocab;  // word embeddings: support j is vocab[sym[j]]
 };
 
 // The CPU loops of centroid_Bregman.c:155-212 with one thread owning one
@@ -675,8 +856,8 @@ __global__ void k_badmm_plain(PlainArgs<T> a) {
     T csum = 0;
     for (int k = 0; k < s; ++k) {
       const size_t e = base + k + (size_t)j * s;
-      const T c = scale_cost<T>(cost_raw<T, CM>(k, j, s, d, a.V, cs, xs, ids, a.dist), rho,
-                                inv_rho);
+      const T c = scale_cost<T>(cost_raw<T, CM>(k, j, s, d, a.V, cs, xs, ids, a.dist,
+                                                a.ctab + base), rho, inv_rho);
       const T t = a.Z[e] * dexp<T>(-(c + a.Y[e])) + (T)1e-9;
       a.X[e] = t;
       csum += t;
@@ -713,8 +894,8 @@ __global__ void k_badmm_plain(PlainArgs<T> a) {
       for (int j = 0; j < m; ++j) {
         const double x = (double)a.X[base + k + (size_t)j * s];
         rx += x;
-        for (int dd = 0; dd < d; ++dd)
-          atomicAdd(&acc[s + k * d + dd], x * (double)xs[j * d + dd]);
+        const T *xj = a.vocab ? a.vocab + (size_t)ids[j] * d : xs + (size_t)j * d;
+        for (int dd = 0; dd < d; ++dd) atomicAdd(&acc[s + k * d + dd], x * (double)xj[dd]);
       }
       atomicAdd(&acc[s + s * d + k], rx);
     }
@@ -819,6 +1000,8 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
   constexpr bool F64 = std::is_same<T, double>::value;
   hipStream_t S = ctx->stream;
   const int N = (int)p->N, K = p->K, s = p->s, d = CM == COST_EUCLID ? p->dim : 0;
+  const int dsup = (CM == COST_EUCLID || CM == COST_TABLE) ? p->dim : 0;  // support dimension
+  const bool word = p->metric == D2MI_WORD_EMBED;
   const int m_max = p->m_max;
   T *Z = F64 ? (T *)p->Z64.p : (T *)p->Z32.p;
   T *Y = F64 ? (T *)p->Y64.p : (T *)p->Y32.p;
@@ -828,7 +1011,44 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
   T *cwT = F64 ? (T *)p->cw64.p : (T *)p->cw32.p;
   T *csT = F64 ? (T *)p->cs64.p : (T *)p->cs32.p;
   const int *sym = p->sym.as<int>();
-  const bool euclid_upd = CM == COST_EUCLID;
+  const T *vocab = F64 ? (const T *)p->vocab64.p : (const T *)p->vocab32.p;
+  const bool euclid_upd = CM == COST_EUCLID || CM == COST_TABLE;
+  T *ctab = nullptr, *xout = nullptr;
+  if constexpr (CM == COST_TABLE) {
+    D2MI_TRY(p->Ctab.ensure(sizeof(T) * (size_t)s * p->col));
+    D2MI_TRY(p->Xout.ensure(sizeof(T) * (size_t)s * p->col));
+    D2MI_TRY(ctx->rxbuf.ensure(sizeof(double) * (size_t)K * s));
+    ctab = p->Ctab.as<T>();
+    xout = p->Xout.as<T>();
+  }
+  // COST_TABLE: rebuild C from the current centroid supports (calculate_distmat)
+  const int tab_G = std::max(1, std::min(N, ctx->num_cu * 4));
+  const int tab_chunk = (N + tab_G - 1) / tab_G;
+  auto build_table = [&](hipStream_t S) {
+    TabArgs<T> ta{};
+    ta.N = N; ta.s = s; ta.d = dsup; ta.chunk = tab_chunk; ta.perm = ctx->perm.as<int>();
+    ta.p_str = p->p_str.as<int>(); ta.cum = p->cum.as<long long>(); ta.label = ctx->label.as<int>();
+    ta.supp = supp; ta.sym = sym; ta.vocab = vocab; ta.csupp = csT; ta.ctab = ctab;
+    const size_t lds = sizeof(T) * (size_t)s * dsup;
+    const int G = (N + tab_chunk - 1) / tab_chunk;
+    if (word) k_badmm_cost_table<T, true><<<G, NT, lds, S>>>(ta);
+    else k_badmm_cost_table<T, false><<<G, NT, lds, S>>>(ta);
+  };
+  // COST_TABLE support update: numerators from X, then supports
+  auto update_supports = [&](hipStream_t S, int *seg, int *flags) -> int {
+    D2MI_TRY(ctx->rec2.ensure(sizeof(double) * (size_t)(tab_G + K) * s * dsup));
+    SuppArgs<T> sa2{};
+    sa2.N = N; sa2.s = s; sa2.d = dsup; sa2.chunk = tab_chunk; sa2.perm = ctx->perm.as<int>();
+    sa2.p_str = p->p_str.as<int>(); sa2.cum = p->cum.as<long long>(); sa2.label = ctx->label.as<int>();
+    sa2.supp = supp; sa2.sym = sym; sa2.vocab = vocab; sa2.X = xout; sa2.rec = ctx->rec2.as<double>();
+    const int G = (N + tab_chunk - 1) / tab_chunk;
+    if (word) k_badmm_supp_num<T, true><<<G, NT, 0, S>>>(sa2);
+    else k_badmm_supp_num<T, false><<<G, NT, 0, S>>>(sa2);
+    k_badmm_supp_fin<T><<<K, NT, 0, S>>>(s, dsup, tab_chunk, seg, ctx->rec2.as<double>(),
+                                         ctx->rxbuf.as<double>(), p->cs64.as<double>(), csT, flags);
+    D2MI_HIP(hipGetLastError());
+    return D2MI_OK;
+  };
 
   // ---- scratch
   const int nb = (N + NT - 1) / NT;
@@ -866,6 +1086,7 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
 
   // ---- 2. setup
   D2MI_HIP(hipEventRecord(ctx->ev[0], S));
+  if constexpr (CM == COST_TABLE) build_table(S);
   const int setup_per_block = std::max(1, N / (ctx->num_cu * 8));
   const int setup_blocks = (N + setup_per_block - 1) / setup_per_block;
   D2MI_TRY(ctx->part.ensure(sizeof(double) * (size_t)std::max(setup_blocks, 3 * 65536)));
@@ -876,7 +1097,7 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
     sa.lsw = ctx->lsw.as<char>(); sa.force_init = fresh ? 1 : 0;
     sa.write_state = variant == D2MI_BADMM_PLAIN ? 1 : 0;
     sa.w = w; sa.supp = supp; sa.sym = sym; sa.dist = dist; sa.cw = cwT; sa.csupp = csT;
-    sa.Z = Z; sa.Y = Y; sa.part = ctx->part.as<double>();
+    sa.Z = Z; sa.Y = Y; sa.ctab = ctab; sa.part = ctx->part.as<double>();
     const size_t lds = align16(sizeof(T) * ((size_t)m_max * std::max(d, 1) + (size_t)s * std::max(d, 1)) +
                                sizeof(int) * m_max) + 8 * sizeof(double);
     k_badmm_setup<T, CM><<<setup_blocks, NT, lds, S>>>(sa);
@@ -949,6 +1170,7 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
       pa.p_str = p->p_str.as<int>(); pa.cum = p->cum.as<long long>(); pa.perm = ctx->perm.as<int>();
       pa.label = ctx->label.as<int>(); pa.w = w; pa.supp = supp; pa.sym = sym; pa.dist = dist;
       pa.cw = cwT; pa.csupp = csT; pa.scal = scal; pa.Z = Z; pa.Y = Y;
+      pa.ctab = ctab; pa.xout = xout;
       pa.rec = ctx->rec.as<double>(); pa.resid_part = ctx->part.as<double>();
       pa.first = iter == 0; pa.force_init = fresh ? 1 : 0; pa.lsw = ctx->lsw.as<char>();
       D2MI_HIP(hipEventRecord(evs[2 * iter], S));
@@ -965,9 +1187,14 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
       fa.resid = resid; fa.iter = iter; fa.seg = seg; fa.rec = ctx->rec.as<double>();
       fa.resid_part = ctx->part.as<double>(); fa.cw64 = p->cw64.as<double>(); fa.cwT = cwT;
       fa.cs64 = p->cs64.as<double>(); fa.csT = csT; fa.resid_out = ctx->resid.as<double>();
+      fa.rxout = CM == COST_TABLE ? ctx->rxbuf.as<double>() : nullptr;
       fa.flags = flags;
       k_badmm_finalize<T><<<K, NT, sizeof(double) * (R + 8), S>>>(fa);
       D2MI_HIP(hipGetLastError());
+      if (CM == COST_TABLE && upd) {  // step 5 (:196-227): supports, then C again
+        D2MI_TRY(update_supports(S, seg, flags));
+        build_table(S);
+      }
       ++n_launch;
     }
   } else {
@@ -975,7 +1202,7 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
     const size_t n = (size_t)s * p->col;
     DevMem &Xm = F64 ? ctx->X64 : ctx->X32;
     D2MI_TRY(Xm.ensure(n * sizeof(T)));
-    const size_t accn = (size_t)K * s * (d + 2);
+    const size_t accn = (size_t)K * s * (dsup + 2);
     D2MI_TRY(ctx->rec.ensure(sizeof(double) * accn));
     k_zero_f64<<<256, NT, 0, S>>>(ctx->rec.as<double>(), accn);
     evs.resize(2 * (maxIters + 1));
@@ -983,17 +1210,19 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
     for (int iter = 0; iter <= maxIters; ++iter) {
       const int upd = euclid_upd && (iter % updatePerLoops == 0);
       PlainArgs<T> pa{};
-      pa.N = N; pa.s = s; pa.d = d; pa.V = p->V; pa.upd = upd;
+      pa.N = N; pa.s = s; pa.d = dsup; pa.V = p->V; pa.upd = upd;
+      pa.ctab = ctab; pa.vocab = word ? vocab : nullptr;
       pa.p_str = p->p_str.as<int>(); pa.cum = p->cum.as<long long>(); pa.label = ctx->label.as<int>();
       pa.w = w; pa.supp = supp; pa.sym = sym; pa.dist = dist; pa.cw = cwT; pa.csupp = csT;
       pa.scal = scal; pa.Z = Z; pa.Y = Y; pa.X = (T *)Xm.p; pa.acc = ctx->rec.as<double>();
       D2MI_HIP(hipEventRecord(evs[2 * iter], S));
       k_badmm_plain<T, CM><<<(N + 127) / 128, 128, 0, S>>>(pa);
       D2MI_HIP(hipEventRecord(evs[2 * iter + 1], S));
-      k_badmm_plain_finalize<T><<<K, NT, 0, S>>>(K, s, d, upd, ctx->rec.as<double>(),
+      k_badmm_plain_finalize<T><<<K, NT, 0, S>>>(K, s, dsup, upd, ctx->rec.as<double>(),
                                                  p->cw64.as<double>(), cwT,
                                                  p->cs64.as<double>(), csT, flags);
       D2MI_HIP(hipGetLastError());
+      if (CM == COST_TABLE && upd) build_table(S);
       ++n_launch;
     }
   }
@@ -1061,13 +1290,20 @@ extern "C" int d2mi_badmm(d2mi_ctx *ctx, int phase, int maxIters, double rhoCoef
     d2mi_set_error("d2mi_badmm: bad options");
     return D2MI_EINVAL;
   }
-  if (p->metric == D2MI_WORD_EMBED) {
-    d2mi_set_error("word-embedding B-ADMM is not built yet");
-    return D2MI_EUNSUPPORTED;
-  }
-  if (p->metric == D2MI_EUCLIDEAN && p->dim > MAXD_FUSED) {
-    d2mi_set_error("Euclidean B-ADMM handles d <= %d in this build", MAXD_FUSED);
-    return D2MI_EUNSUPPORTED;
+  // stored-cost path: word embeddings, Euclidean d > MAXD_FUSED, or on request
+  const char *force_env = getenv("D2MI_BADMM_TABLE");
+  const bool force_table = force_env && atoi(force_env) != 0;
+  const bool table = p->metric == D2MI_WORD_EMBED ||
+                     (p->metric == D2MI_EUCLIDEAN && (p->dim > MAXD_FUSED || force_table));
+  if (table) {
+    if (p->s > 4 * SUP_KPT) {
+      d2mi_set_error("stored-cost B-ADMM handles s <= %d", 4 * SUP_KPT);
+      return D2MI_EUNSUPPORTED;
+    }
+    if ((size_t)p->s * p->dim * (precision == 64 ? 8 : 4) > ctx->lds_max) {
+      d2mi_set_error("centroid supports of %d x %d do not fit LDS", p->s, p->dim);
+      return D2MI_EUNSUPPORTED;
+    }
   }
   if (p->K > 16384) {
     d2mi_set_error("K = %d too large", p->K);
@@ -1076,6 +1312,11 @@ extern "C" int d2mi_badmm(d2mi_ctx *ctx, int phase, int maxIters, double rhoCoef
   const int fresh = d2mi_ensure_plans(ctx, p, precision);
   if (fresh < 0) return fresh;
   const bool f = fresh == 1;
+  if (table) {
+    if (precision == 64)
+      return run_badmm<double, COST_TABLE>(ctx, p, maxIters, rhoCoeff, updatePerLoops, variant, stats, f);
+    return run_badmm<float, COST_TABLE>(ctx, p, maxIters, rhoCoeff, updatePerLoops, variant, stats, f);
+  }
   if (precision == 64) {
     switch (p->metric) {
       case D2MI_EUCLIDEAN:

@@ -58,6 +58,9 @@ struct PhaseDev {
   // resident transport plans, element (k, j) of object i at s*cum[i] + k + j*s
   DevMem Z32, Y32, Z64, Y64;
   int plan_s32 = 0, plan_s64 = 0;  // s the plans were sized for (0 = none)
+  // stored ground costs and the X of support-update passes (plan layout),
+  // for phases whose cost is not recomputed inside the plan kernel
+  DevMem Ctab, Xout;
 };
 
 struct d2mi_ctx {
@@ -68,6 +71,7 @@ struct d2mi_ctx {
   std::vector<PhaseDev> ph;
   // B-ADMM scratch
   DevMem perm, blkcnt, segstart, rec, part, scal, flags, resid, X32, X64;
+  DevMem rec2, rxbuf;           // support-numerator records, X row sums per label
   // pdist2 scratch
   DevMem pA, pB, pC;
   // labeling / Elkan state (clustering.h:106-112), resident in HBM

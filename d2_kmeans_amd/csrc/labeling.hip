@@ -255,11 +255,13 @@ int lp_over_phases(d2mi_ctx *ctx, int selected_phase, Pairing pairing, int njobs
       case D2MI_SPARSE_HISTOGRAM:
         a.cost_mode = LP_COST_SPARSE;
         break;
-      default:
-        d2mi_set_error("exact distances for word-embedding phases are not built yet");
-        return D2MI_EUNSUPPORTED;
+      default:  // word embeddings (clustering.c:63-80, _dpdist2_sym)
+        a.cost_mode = LP_COST_WORD;
+        a.d = P.dim;
+        a.vocab = P.vocab64.as<double>();
+        break;
     }
-    const bool euclid = P.metric == D2MI_EUCLIDEAN;
+    const bool euclid = P.metric == D2MI_EUCLIDEAN || P.metric == D2MI_WORD_EMBED;
     a.sA = P.s;
     a.A_supp = euclid ? (pairing == OLD_NEW ? P.cs_old.as<double>() : P.cs64.as<double>()) : nullptr;
     a.A_w = pairing == OLD_NEW ? P.cw_old.as<double>() : P.cw64.as<double>();
@@ -269,8 +271,9 @@ int lp_over_phases(d2mi_ctx *ctx, int selected_phase, Pairing pairing, int njobs
       a.p_str = P.p_str.as<int>();
       a.cum = P.cum.as<long long>();
       a.w = P.w64.as<double>();
-      a.supp = euclid ? P.supp64.as<double>() : nullptr;
-      a.sym = P.metric == D2MI_SPARSE_HISTOGRAM ? P.sym.as<int>() : nullptr;
+      a.supp = P.metric == D2MI_EUCLIDEAN ? P.supp64.as<double>() : nullptr;
+      a.sym = (P.metric == D2MI_SPARSE_HISTOGRAM || P.metric == D2MI_WORD_EMBED)
+                  ? P.sym.as<int>() : nullptr;
       a.m_max = P.m_max;
     } else {
       a.b_is_table = 1;

@@ -49,6 +49,7 @@ enum CostMode {
   COST_HIST = 1,    // dense histogram ground cost dist[k + j*V]
   COST_SPARSE = 2,  // sparse histogram: dist[ids[j]*V + k]
   COST_GLOBAL = 3,  // precomputed FP64 costs g[k + j*n]
+  COST_WORD = 4,    // word embeddings: A row k against vocab row ids[j] (B = vocab)
 };
 
 enum TableMode {
@@ -102,6 +103,15 @@ struct Cost {
     switch (mode) {
       case COST_EUCLID: {
         const double *a = A + (size_t)k * d, *b = B + (size_t)j * d;
+        double acc = 0.0;
+        for (int t = 0; t < d; ++t) {
+          const double u = a[t] - b[t];
+          acc += u * u;
+        }
+        return acc;
+      }
+      case COST_WORD: {
+        const double *a = A + (size_t)k * d, *b = B + (size_t)ids[j] * d;
         double acc = 0.0;
         for (int t = 0; t < d; ++t) {
           const double u = a[t] - b[t];

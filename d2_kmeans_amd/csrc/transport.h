@@ -8,6 +8,7 @@ enum LpCostMode {
   LP_COST_HIST = 1,    // dense histogram ground cost dist[k + j*V]
   LP_COST_SPARSE = 2,  // sparse histogram: dist[id_j * V + k]
   LP_COST_GLOBAL = 3,  // precomputed costs gcost[job*gstride + k + j*n]
+  LP_COST_WORD = 4,    // word embeddings: centroid rows against vocab[sym[j]]
 };
 
 // One LP per job: rows = the sA supports of entry job_a[job] of centroid
@@ -36,6 +37,7 @@ struct LpArgs {
   const double *B_w;
   int V;
   const double *dist;
+  const double *vocab;  // LP_COST_WORD: [V*d]
   const double *gcost;
   long long gstride;
   int n_max, m_max;

@@ -14,10 +14,14 @@ namespace {
 __global__ __launch_bounds__(1024) void k_transport(LpArgs a) {
   extern __shared__ __align__(16) unsigned char lds[];
   const int tid = threadIdx.x, T = blockDim.x;
-  const bool euclid = a.cost_mode == LP_COST_EUCLID;
-  const int dA = euclid && a.stage_a ? a.d : 0, dB = euclid && a.stage_b ? a.d : 0;
+  // rows carry supports for Euclidean and word phases; columns for Euclidean
+  // objects, or for the other centroid table of a centroid pair
+  const bool word = a.cost_mode == LP_COST_WORD;
+  const bool euclid = a.cost_mode == LP_COST_EUCLID || (word && a.b_is_table);
+  const bool rows = euclid || word;
+  const int dA = rows && a.stage_a ? a.d : 0, dB = euclid && a.stage_b ? a.d : 0;
   const lp::Layout L(a.n_max, a.m_max, dA, dB, a.table != lp::TAB_NONE,
-                     a.cost_mode == LP_COST_SPARSE);
+                     a.cost_mode == LP_COST_SPARSE || word);
   double *A = (double *)(lds + L.A), *B = (double *)(lds + L.B);
   double *wa = (double *)(lds + L.wa), *wb = (double *)(lds + L.wb);
   int *ids = (int *)(lds + L.ids);
@@ -27,7 +31,7 @@ __global__ __launch_bounds__(1024) void k_transport(LpArgs a) {
     const int ka = a.implicit_K > 0 ? job % a.implicit_K : a.job_a[job];
     const int jb = a.implicit_K > 0 ? job / a.implicit_K : a.job_b[job];
     const int n = a.sA;
-    const double *Asrc = euclid ? a.A_supp + (size_t)ka * n * a.d : nullptr;
+    const double *Asrc = rows ? a.A_supp + (size_t)ka * n * a.d : nullptr;
     const double *wA = a.A_w + (size_t)ka * n;
     int m;
     const double *Bsrc = nullptr, *wB;
@@ -39,7 +43,7 @@ __global__ __launch_bounds__(1024) void k_transport(LpArgs a) {
     } else {
       m = a.p_str[jb];
       const long long off = a.cum[jb];
-      Bsrc = euclid ? a.supp + (size_t)off * a.d : nullptr;
+      Bsrc = euclid ? a.supp + (size_t)off * a.d : (word ? a.vocab : nullptr);
       wB = a.w + off;
       idsrc = a.sym ? a.sym + off : nullptr;
     }
@@ -51,11 +55,12 @@ __global__ __launch_bounds__(1024) void k_transport(LpArgs a) {
     if (dB > 0)
       for (int q = tid; q < m * dB; q += T) B[q] = Bsrc[q];
     int mode = a.cost_mode;
-    if (mode == LP_COST_SPARSE) {
+    if (mode == LP_COST_SPARSE || mode == LP_COST_WORD) {
       if (idsrc) {
         for (int q = tid; q < m; q += T) ids[q] = idsrc[q];
       } else {
-        mode = LP_COST_HIST;  // two centroids: both are full V-bin histograms
+        // two centroids: full V-bin histograms, or two sets of dense vectors
+        mode = mode == LP_COST_SPARSE ? LP_COST_HIST : LP_COST_EUCLID;
       }
     }
     __syncthreads();
@@ -108,8 +113,10 @@ static int env_int(const char *name, int dflt) {
 }
 
 int d2mi_plan_transport(d2mi_ctx *ctx, LpArgs &a) {
-  const bool euclid = a.cost_mode == LP_COST_EUCLID;
-  const bool sparse = a.cost_mode == LP_COST_SPARSE;
+  const bool word = a.cost_mode == LP_COST_WORD;
+  const bool euclid = a.cost_mode == LP_COST_EUCLID || (word && a.b_is_table);
+  const bool rows = euclid || word;
+  const bool sparse = a.cost_mode == LP_COST_SPARSE || word;
   const size_t nn = (size_t)a.n_max + a.m_max;
   if (nn > 32767 || (size_t)a.n_max * a.m_max >= (1u << 24)) {
     d2mi_set_error("transport LP of %d x %d is beyond the solver's size limit", a.n_max, a.m_max);
@@ -119,11 +126,11 @@ int d2mi_plan_transport(d2mi_ctx *ctx, LpArgs &a) {
   // row supports, then the column supports
   const size_t cap = ctx->lds_max;
   auto bytes = [&](bool tab, bool sa, bool sb) {
-    return lp::Layout(a.n_max, a.m_max, euclid && sa ? a.d : 0, euclid && sb ? a.d : 0, tab, sparse)
+    return lp::Layout(a.n_max, a.m_max, rows && sa ? a.d : 0, euclid && sb ? a.d : 0, tab, sparse)
         .total;
   };
   bool tab = bytes(true, false, false) <= cap && env_int("D2MI_LP_TABLE", 1) != 0;
-  bool sa = euclid && bytes(tab, true, false) <= cap;
+  bool sa = rows && bytes(tab, true, false) <= cap;
   bool sb = euclid && bytes(tab, sa, true) <= cap && (size_t)a.m_max * a.d <= 4096;
   if (bytes(tab, sa, sb) > cap) {
     d2mi_set_error("transport LP of %d x %d needs %zu B of LDS (limit %zu)", a.n_max, a.m_max,

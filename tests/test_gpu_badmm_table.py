@@ -1,0 +1,109 @@
+"""B-ADMM with stored ground costs (Euclidean d > 8, word embeddings) and
+exact distances for word-embedding phases, against the FP64 oracle (which
+the CPU suite pins to the reference build for both kinds of phase).
+
+Tolerances as for the fused path: FP64 to 1e-9 relative, the FP32 objective
+within 1e-5 of FP64."""
+import os
+
+import numpy as np
+import pytest
+
+from d2_kmeans_amd import synth
+from d2_kmeans_amd.data import BADMMOptions
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(1e-300, np.max(np.abs(b))))
+
+
+def device_update(engine, data, lab, lsw, c0, opts, precision):
+    engine.set_data(data)
+    engine.set_centroids(0, c0)
+    engine.set_labels(lab, lsw)
+    st = engine.badmm(0, opts, precision=precision)
+    return st, engine.get_centroids(0)
+
+
+def case(kind):
+    if kind == "word_embed":
+        data, _ = synth.word_embed_docs(n=600, V=500, d=24, m_lo=5, m_hi=40, seed=17, s=8)
+    else:
+        data, _ = synth.serving_requests(n=600, d=64, m_lo=6, m_hi=90, seed=18)
+    ph = data.phases[0]
+    c0 = O.centroid_rands(ph, 5, 8, seed=0)
+    lab, lsw, _, _ = O.labeling(data, [c0])
+    assert len(np.unique(lab)) == 5
+    return data, ph, c0, lab, lsw
+
+
+@pytest.mark.parametrize("kind", ["word_embed", "euclid_d64"])
+def test_stored_cost_update_matches_oracle(engine, kind):
+    say(f"stored-cost B-ADMM vs oracle: {kind}")
+    data, ph, c0, lab, lsw = case(kind)
+    opts = BADMMOptions(12, 2.0, 5)
+    st64, c64 = device_update(engine, data, lab, lsw, c0, opts, 64)
+    co = c0.copy()
+    ro = O.badmm(ph, lab, np.ones(ph.size, np.int8), co, np.zeros(c0.s * ph.col), opts)
+    assert rel(c64.p_w, co.p_w) < 1e-9
+    assert rel(c64.p_supp, co.p_supp) < 1e-9
+    assert abs(st64["obj"] - ro["obj"]) <= 1e-9 * abs(ro["obj"])
+    st32, c32 = device_update(engine, data, lab, lsw, c0, opts, 32)
+    assert abs(st32["obj"] - st64["obj"]) <= 1e-5 * abs(st64["obj"])
+    assert rel(c32.p_supp, c64.p_supp) < 1e-4
+
+
+def test_stored_and_fused_costs_agree(engine):
+    """The same Euclidean d = 3 update with costs recomputed in the plan
+    kernel and with the stored table (D2MI_BADMM_TABLE=1)."""
+    say("stored vs fused costs, d=3")
+    data, _ = synth.gaussian_mixture(n=800, seed=23)
+    ph = data.phases[0]
+    c0 = O.centroid_rands(ph, 6, 6, seed=0)
+    lab, lsw, _, _ = O.labeling(data, [c0])
+    opts = BADMMOptions(15, 2.0, 5)
+    sa, ca = device_update(engine, data, lab, lsw, c0, opts, 64)
+    os.environ["D2MI_BADMM_TABLE"] = "1"
+    try:
+        sb, cb = device_update(engine, data, lab, lsw, c0, opts, 64)
+    finally:
+        os.environ.pop("D2MI_BADMM_TABLE", None)
+    assert rel(cb.p_w, ca.p_w) < 1e-11
+    assert rel(cb.p_supp, ca.p_supp) < 1e-11
+    assert abs(sb["obj"] - sa["obj"]) <= 1e-11 * abs(sa["obj"])
+
+
+def test_word_embed_distances_and_labels(engine):
+    say("word-embedding exact distances and labeling")
+    data, ph, c0, lab, lsw = case("word_embed")
+    engine.set_data(data)
+    engine.set_centroids(0, c0)
+    engine.set_labels(np.zeros(ph.size, np.int32), np.zeros(ph.size, np.int8))
+    obj = np.repeat(np.arange(40), c0.K).astype(np.int32)
+    cen = np.tile(np.arange(c0.K), 40).astype(np.int32)
+    got = engine.distances(obj, cen)
+    ref = np.array([O.distance(data, int(i), [c0], int(j)) for i, j in zip(obj, cen)])
+    np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-13)
+    st = engine.labeling(use_elkan=False)
+    np.testing.assert_array_equal(engine.get_labels()[0], lab)
+
+
+def test_word_embed_clustering_matches_oracle(engine):
+    say("word-embedding clustering loop vs oracle")
+    data, ph, c0, lab, lsw = case("word_embed")
+    K = c0.K
+    opts = BADMMOptions(10, 2.0, 10)
+    engine.set_data(data)
+    engine.set_centroids(0, c0)
+    res = engine.clustering(K, max_iter=6, opts=opts, use_triangle=True, precision=64)
+    co = [c0.copy()]
+    lab_o, it, _, cost = O.clustering(data, co, K, 6, opts, use_triangle=True)
+    np.testing.assert_array_equal(engine.get_labels()[0], lab_o)
+    assert rel(engine.get_centroids(0).p_supp, co[0].p_supp) < 1e-8

@@ -89,6 +89,31 @@ def test_badmm_matches_reference_live(seed, K, s, iters, upl):
 
 
 @needs_ref
+@pytest.mark.parametrize("kind", ["word_embed", "euclid_d16"])
+def test_badmm_stored_cost_phases_match_reference_live(kind):
+    """Word embeddings (centroid_Bregman.c:229-263, _dpdist2_sym) and a
+    Euclidean phase with d = 16: the restatement against the reference build."""
+    if kind == "word_embed":
+        data, _ = synth.word_embed_docs(n=200, V=300, d=10, m_lo=5, m_hi=30, seed=7, s=6)
+    else:
+        data, _ = synth.serving_requests(n=200, d=16, m_lo=6, m_hi=40, seed=8)
+    ph = data.phases[0]
+    K, s = 4, 6
+    c0 = O.centroid_rands(ph, K, s, seed=0)
+    lab, lsw, _, _ = O.labeling(data, [c0])
+    if len(np.unique(lab)) < K:
+        pytest.skip("empty cluster")
+    opts = BADMMOptions(12, 2.0, 5)
+    ca, cb = c0.copy(), c0.copy()
+    Za, Zb = np.zeros(s * ph.col), np.zeros(s * ph.col)
+    O.badmm(ph, lab, lsw, ca, Za, opts)
+    O.ref_badmm(ph, lab, lsw, cb, Zb, opts)
+    np.testing.assert_allclose(ca.p_w, cb.p_w, rtol=1e-10, atol=1e-14)
+    np.testing.assert_allclose(ca.p_supp, cb.p_supp, rtol=1e-10, atol=1e-12)
+    np.testing.assert_allclose(Za, Zb, rtol=1e-9, atol=1e-15)
+
+
+@needs_ref
 def test_merge_matches_reference_live():
     rng = np.random.default_rng(0)
     for m, n in [(12, 6), (9, 2), (40, 39), (80, 40)]:
