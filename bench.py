@@ -122,11 +122,20 @@ def assign_leg(eng, args):
     from d2_kmeans_amd import synth
     from d2_kmeans_amd.serve import Server
     K, s, d = 64, 48, 64
-    log(f"[assign] training set and K={K} centroids of {s} points in d={d}")
-    train, _ = synth.serving_requests(n=4000, seed=synth.SEED_BASE + 40)
-    cent = synth.seed_centroids(train.phases[0], K, s)
+    log(f"[assign] training K={K} centroids of {s} points in d={d} on {args.assign_train} "
+        "queries with the clustering loop (FP32 B-ADMM, Elkan labeling)")
+    from d2_kmeans_amd.data import BADMMOptions
+    train, tmeta = synth.serving_requests(n=args.assign_train, seed=synth.SEED_BASE + 40)
     eng.set_data(train)
-    eng.set_centroids(0, cent)
+    # one seed per mixture component (an object of that component with >= s points)
+    eng.set_centroids(0, synth.seed_centroids(train.phases[0], K, s, labels=tmeta["dominant"]))
+    t0 = time.perf_counter()
+    tr = eng.clustering(K, max_iter=args.assign_train_iters, opts=BADMMOptions(10, 2.0, 10),
+                        use_triangle=True, precision=32)
+    train_s = time.perf_counter() - t0
+    cent = eng.get_centroids(0)
+    log(f"[assign] trained in {train_s:.1f} s: {tr['iterations']} iterations, "
+        f"changes {tr['changes']}")
     qd, qmeta = synth.serving_requests(n=args.assign_n, seed=synth.SEED_BASE + 41)
     qph = qd.phases[0]
     Q = [qph.object(i) for i in range(qph.size)]
@@ -159,8 +168,10 @@ def assign_leg(eng, args):
     res = {
         "workload": f"K={K} centroids x {s} points, d={d}; queries m~lognormal(ln 50, 1) in "
                     f"[5,500] (mean {qph.p_str.mean():.1f}), Dirichlet(1) weights, 64-component "
-                    "Gaussian mixture; centroids seeded from a 4000-query training set "
-                    "(first objects with >= 48 points, cut to their 48 heaviest)",
+                    f"Gaussian mixture; centroids trained on {args.assign_train} queries of the "
+                    f"same law by the clustering loop ({tr['iterations']} iterations, FP32 "
+                    "B-ADMM, Elkan labeling)",
+        "train_seconds": train_s,
         "queries": int(qph.size), "qps_offered": args.assign_qps,
         "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
         "qps_achieved": qph.size / el, "burst_qps": burst,
@@ -193,6 +204,8 @@ def main():
     ap.add_argument("--assign-qps", type=float, default=1000.0)
     ap.add_argument("--assign-plain-n", type=int, default=300)
     ap.add_argument("--assign-check", type=int, default=100)
+    ap.add_argument("--assign-train", type=int, default=4000)
+    ap.add_argument("--assign-train-iters", type=int, default=6)
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run one FP32->FP64 plan copy kernel (reads 4 B, writes 8 B per "
                          "plan element) to scale rocprof byte counters")

@@ -31,6 +31,11 @@ constexpr int NT = 256;        // threads per workgroup (4 waves of 64)
 constexpr int MAXD_FUSED = 8;  // Euclidean cost fused on the fly for d <= 8
 constexpr int FLAG_BADLABEL = 1, FLAG_EMPTY = 2, FLAG_NUMERIC = 4;
 
+inline bool getenv_flag(const char *name) {
+  const char *e = getenv(name);
+  return e && atoi(e) != 0;
+}
+
 // COST_TABLE: the ground cost is read from a stored table C[s*cum[i] + k + j*s]
 // (the reference's own C array, clustering_util.c:179), rebuilt after every
 // support update; used for Euclidean d > MAXD_FUSED and for word embeddings.
@@ -65,8 +70,9 @@ __host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~size_t(
 // LDS layout of the plan kernel, shared by host (sizing) and device.
 struct PlanLds {
   size_t A, B, X, w, xs, cs, cw, g, rr, rp, ids, acc, red, up, total;
-  __host__ __device__ PlanLds(int tsize, int sp, int m_max, int s, int d, int keepx, int R) {
-    const size_t EP = (size_t)sp * m_max;
+  __host__ __device__ PlanLds(int tsize, int sp, int m_max, int s, int d, int keepx, int R,
+                              bool gm = false) {
+    const size_t EP = gm ? 0 : (size_t)sp * m_max;  // gm: plan scratch lives in HBM
     size_t o = 0;
     A = o; o = align16(o + EP * tsize);
     B = o; o = align16(o + EP * tsize);
@@ -107,6 +113,8 @@ struct PlanArgs {
   T *__restrict__ Y;
   const T *__restrict__ ctab;       // COST_TABLE: stored costs (plan layout)
   T *__restrict__ xout;             // COST_TABLE, support-update passes: X out (plan layout)
+  T *__restrict__ xg;               // HBM-scratch kernel: X (plan layout)
+  T *__restrict__ z0g;              // HBM-scratch kernel, residual passes: Z before the pass
   double *__restrict__ rec;         // [(G + K) * R]
   double *__restrict__ resid_part;  // [G * 3]
 };
@@ -164,7 +172,12 @@ constexpr int PRE_X = 4;  // register slots for supports: m_max * d <= 1024
 //   pass 3 (four waves): partial row sums, combined by wave 0 into the
 //          normalisers and the per-label sums;
 //   pass 4 (element per thread): z = z' / rowsum * c_w, Y += x - z, stored.
-template <typename T, int CM, int RMAX, int DIM>
+//
+// GM (objects whose plan does not fit LDS): the same passes with the plan
+// scratch in HBM -- t and z' in place in Z, y (+ x) in place in Y, x in a
+// scratch array; the workgroup barriers order these accesses as they order
+// the LDS ones.
+template <typename T, int CM, int RMAX, int DIM, bool GM = false>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(D2MI_PLAN_WAVES_PER_EU)))
 void k_badmm_plan(PlanArgs<T> a, const int *__restrict__ g_perm, const int *__restrict__ g_str,
                   const long long *__restrict__ g_cum, const int *__restrict__ g_label,
@@ -172,9 +185,9 @@ void k_badmm_plan(PlanArgs<T> a, const int *__restrict__ g_perm, const int *__re
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr bool F64 = std::is_same<T, double>::value;
   constexpr int RS = RMAX > 0 ? RMAX : 1;
-  const int s = a.s, d = DIM > 0 ? DIM : a.d, sp = a.sp, tid = threadIdx.x;
+  const int s = a.s, d = DIM > 0 ? DIM : a.d, sp = GM ? a.s : a.sp, tid = threadIdx.x;
   const int lane = tid & 63, wv = tid >> 6;
-  const PlanLds L(sizeof(T), sp, a.m_max, s, d, a.keepx, a.R);
+  const PlanLds L(sizeof(T), a.sp, a.m_max, s, d, a.keepx, a.R, GM);
   T *A = (T *)(smem + L.A), *B = (T *)(smem + L.B), *Xs = (T *)(smem + L.X);
   T *wl = (T *)(smem + L.w), *xs = (T *)(smem + L.xs);
   T *cs = (T *)(smem + L.cs), *cwl = (T *)(smem + L.cw), *g = (T *)(smem + L.g);
@@ -200,6 +213,11 @@ void k_badmm_plan(PlanArgs<T> a, const int *__restrict__ g_perm, const int *__re
     const long long off = g_cum[o];
     const int E = s * m;
     const size_t base = (size_t)s * off;
+    if constexpr (GM) {
+      A = a.Z + base;
+      B = a.Y + base;
+      Xs = a.xg + base;
+    }
     // first pass of an update: Y = 0 (:128) and, for objects whose label
     // switched, Z = w_j * c_w[label][k] (:116-124), without touching HBM
     const bool reinit = a.first && (a.force_init || g_lsw[o] == 1);
@@ -288,6 +306,7 @@ void k_badmm_plan(PlanArgs<T> a, const int *__restrict__ g_perm, const int *__re
           const T z = reinit ? wl[j] * cwl[k] : a.Z[base + e];
           const T y = a.first ? (T)0 : a.Y[base + e];
           const int li = k + j * sp;
+          if (GM && a.resid) a.z0g[base + e] = z;
           A[li] = z * dexp<T>(-(c + y)) + (T)1e-9;
           B[li] = y;
           k += dk;
@@ -405,14 +424,14 @@ void k_badmm_plan(PlanArgs<T> a, const int *__restrict__ g_perm, const int *__re
               cost_raw<T, CM, DIM>(k, j, s, d, a.V, cs, xs, ids, a.dist, a.ctab + base), rho,
               inv_rho);
           const T x = Xs[li];
-          const T z0 = reinit ? wl[j] * cwl[k] : a.Z[base + e];
+          const T z0 = GM ? a.z0g[base + e] : (reinit ? wl[j] * cwl[k] : a.Z[base + e]);
           r_obj += (double)c * (double)x;
           r_pr += fabs((double)x - (double)z);
           r_dr += fabs((double)z0 - (double)z);
         }
         a.Z[base + e] = z;
         a.Y[base + e] = a.keepx ? B[li] + (Xs[li] - z) : B[li] - z;
-        if (CM == COST_TABLE && a.upd) a.xout[base + e] = Xs[li];  // for the support GEMM
+        if (CM == COST_TABLE && a.upd && !GM) a.xout[base + e] = Xs[li];  // for the support GEMM
         k += dk;
         j += dj;
         if (k >= s) { k -= s; ++j; }
@@ -970,17 +989,22 @@ namespace {
 
 // resident workgroups per CU of the plan kernel at this LDS size (HIP's
 // occupancy calculator), so one launch is exactly one round of workgroups
-template <typename T, int CM, int DIM>
+template <typename T, int CM, int DIM, bool GM = false>
 int plan_occupancy(size_t lds) {
   int n = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_badmm_plan<T, CM, 0, DIM>, NT, lds) !=
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_badmm_plan<T, CM, 0, DIM, GM>, NT, lds) !=
           hipSuccess || n <= 0)
     n = 1;
   return n;
 }
 
 template <typename T, int CM, int DIM>
-void launch_plan(int rmax, int G, size_t lds, hipStream_t S, const PlanArgs<T> &pa) {
+void launch_plan(int rmax, int G, size_t lds, hipStream_t S, const PlanArgs<T> &pa, bool gm) {
+  if (gm) {
+    k_badmm_plan<T, CM, 0, DIM, true><<<G, NT, lds, S>>>(pa, pa.perm, pa.p_str, pa.cum, pa.label,
+                                                         pa.lsw);
+    return;
+  }
 #ifndef D2MI_PLAN_EARLY_LOADS
   // Measured on MI355X: holding a whole object's Z/Y in registers costs more
   // occupancy than it saves in latency, so by default loads stay in pass 1.
@@ -1123,9 +1147,10 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
       const char *e = getenv("D2MI_PLAN_WG_PER_CU");
       return e ? atoi(e) : 0;
     }();
-    auto geometry = [&](size_t lds, int *G_out, int *chunk_out) {
+    auto geometry = [&](size_t lds, int *G_out, int *chunk_out, bool g = false) {
       int per_cu;
       if (wg_env > 0) per_cu = wg_env;
+      else if (g) per_cu = plan_occupancy<T, CM, 0, true>(lds);
       else if (CM == COST_EUCLID && d == 2) per_cu = plan_occupancy<T, CM, 2>(lds);
       else if (CM == COST_EUCLID && d == 3) per_cu = plan_occupancy<T, CM, 3>(lds);
       else per_cu = plan_occupancy<T, CM, 0>(lds);
@@ -1133,8 +1158,16 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
       *chunk_out = std::max(1, (N + Gt - 1) / Gt);
       *G_out = (N + *chunk_out - 1) / *chunk_out;
     };
+    // objects whose plan does not fit LDS run the HBM-scratch form
+    const size_t lds_fused = PlanLds(sizeof(T), sp, m_max, s, d, 1, Rmax).total;
+    const bool gm = lds_fused > ctx->lds_max || getenv_flag("D2MI_BADMM_HBM_SCRATCH");
+    if (gm) {
+      D2MI_TRY(p->Xout.ensure(sizeof(T) * (size_t)s * p->col));
+      D2MI_TRY(p->Z0.ensure(sizeof(T) * (size_t)s * p->col));
+      xout = p->Xout.as<T>();
+    }
     int Gmax = 0, chunk_tmp = 0;
-    geometry(PlanLds(sizeof(T), sp, m_max, s, d, 0, s).total, &Gmax, &chunk_tmp);
+    geometry(PlanLds(sizeof(T), sp, m_max, s, d, 0, s, gm).total, &Gmax, &chunk_tmp, gm);
     D2MI_TRY(ctx->rec.ensure(sizeof(double) * (size_t)(Gmax + K) * Rmax));
     D2MI_TRY(ctx->part.ensure(sizeof(double) * (size_t)std::max(setup_blocks, 3 * Gmax)));
     if (m_max > NT * PRE_W || (size_t)m_max * std::max(d, 1) > (size_t)NT * PRE_X) {
@@ -1144,10 +1177,10 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
     }
     const int slots = (s * m_max + NT - 1) / NT;
     const int rmax = slots <= 8 ? 8 : slots <= 16 ? 16 : slots <= 32 ? 32 : 0;
-    const size_t lds_full = PlanLds(sizeof(T), sp, m_max, s, d, 1, Rmax).total;
+    const size_t lds_full = PlanLds(sizeof(T), sp, m_max, s, d, 1, Rmax, gm).total;
     if (lds_full > ctx->lds_max) {
-      d2mi_set_error("object plan s*m_max = %d x %d needs %zu B of LDS (limit %zu)", s, m_max,
-                     lds_full, ctx->lds_max);
+      d2mi_set_error("B-ADMM with s = %d, d = %d needs %zu B of LDS (limit %zu)", s, d, lds_full,
+                     ctx->lds_max);
       return D2MI_EUNSUPPORTED;
     }
     evs.resize(2 * (maxIters + 1));
@@ -1157,9 +1190,9 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
       const int resid = is_resid(iter);
       const int keepx = F64 || upd || resid;
       const int R = upd ? s * (d + 2) : s;
-      const size_t lds = PlanLds(sizeof(T), sp, m_max, s, d, keepx, R).total;
+      const size_t lds = PlanLds(sizeof(T), sp, m_max, s, d, keepx, R, gm).total;
       int G, chunk;
-      geometry(lds, &G, &chunk);
+      geometry(lds, &G, &chunk, gm);
       if (G > Gmax) {  // records are sized for Gmax workgroups
         chunk = std::max(1, (N + Gmax - 1) / Gmax);
         G = (N + chunk - 1) / chunk;
@@ -1171,15 +1204,17 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
       pa.label = ctx->label.as<int>(); pa.w = w; pa.supp = supp; pa.sym = sym; pa.dist = dist;
       pa.cw = cwT; pa.csupp = csT; pa.scal = scal; pa.Z = Z; pa.Y = Y;
       pa.ctab = ctab; pa.xout = xout;
+      pa.xg = gm ? p->Xout.as<T>() : nullptr; pa.z0g = gm ? p->Z0.as<T>() : nullptr;
       pa.rec = ctx->rec.as<double>(); pa.resid_part = ctx->part.as<double>();
       pa.first = iter == 0; pa.force_init = fresh ? 1 : 0; pa.lsw = ctx->lsw.as<char>();
       D2MI_HIP(hipEventRecord(evs[2 * iter], S));
       if constexpr (CM == COST_EUCLID) {
-        if (d == 2) launch_plan<T, CM, 2>(rmax, G, lds, S, pa);
-        else if (d == 3) launch_plan<T, CM, 3>(rmax, G, lds, S, pa);
-        else launch_plan<T, CM, 0>(rmax, G, lds, S, pa);
+        if (gm) launch_plan<T, CM, 0>(rmax, G, lds, S, pa, true);
+        else if (d == 2) launch_plan<T, CM, 2>(rmax, G, lds, S, pa, false);
+        else if (d == 3) launch_plan<T, CM, 3>(rmax, G, lds, S, pa, false);
+        else launch_plan<T, CM, 0>(rmax, G, lds, S, pa, false);
       } else {
-        launch_plan<T, CM, 0>(rmax, G, lds, S, pa);
+        launch_plan<T, CM, 0>(rmax, G, lds, S, pa, gm);
       }
       D2MI_HIP(hipEventRecord(evs[2 * iter + 1], S));
       FinArgs<T> fa{};

@@ -60,7 +60,7 @@ struct PhaseDev {
   int plan_s32 = 0, plan_s64 = 0;  // s the plans were sized for (0 = none)
   // stored ground costs and the X of support-update passes (plan layout),
   // for phases whose cost is not recomputed inside the plan kernel
-  DevMem Ctab, Xout;
+  DevMem Ctab, Xout, Z0;
 };
 
 struct d2mi_ctx {

@@ -14,11 +14,14 @@
 // its own stream, so groups admitted at different times overlap on the
 // device; finished groups free their stream and memory at the next progress
 // call.  Per group:
-//   1. lower bounds for every (query, centroid): a transport plan moves each
-//      query point's mass to some centroid point and each centroid point's
-//      mass from some query point, so
-//          LP >= max( sum_j b_j min_k C_kj , sum_k a_k min_j C_kj )
-//      (exact FP64 costs by direct differences);
+//   1. lower bounds for every (query, centroid).  W2^2 splits exactly into
+//      the squared distance of the means plus W2^2 of the centred pair
+//      (the cross term of any coupling vanishes), and a transport plan moves
+//      each query point's mass to some centroid point and each centroid
+//      point's mass from some query point, so with centred costs C~
+//          LP >= |mu_x - mu_c|^2 + max( sum_j b_j min_k C~_kj ,
+//                                       sum_k a_k min_j C~_kj )
+//      (FP64, direct differences);
 //   2. exact LPs for every centroid whose bound is within a factor tau of
 //      the smallest bound (the nearest centroid is almost always among them);
 //   3. exact LPs for the centroids still unsolved that the bounds cannot
@@ -71,15 +74,30 @@ __global__ __launch_bounds__(LB_T) void k_srv_bounds(int K, int s, int d,
                                                      const double *__restrict__ supp,
                                                      double *__restrict__ lb,
                                                      double *__restrict__ vals) {
-  extern __shared__ __align__(16) double shA[];  // s*d, then 4 waves x LB_KC row minima
+  // LDS: s*d centroid supports, 4 waves x LB_KC row minima, d mean shifts
+  extern __shared__ __align__(16) double shA[];
   double *rmin_w = shA + (size_t)s * d;
+  double *delta = rmin_w + 4 * LB_KC;
   __shared__ double red[8];
   const int r = blockIdx.x / K, c = blockIdx.x - (blockIdx.x / K) * K;
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const int m = p_str[r];
   const long long off = cum[r];
   const double *A = cs + (size_t)c * s * d;
+  const double *aw = cw + (size_t)c * s;
   for (int q = tid; q < s * d; q += LB_T) shA[q] = A[q];
+  // delta = mu_c - mu_x, one dimension per thread
+  for (int t = tid; t < d; t += LB_T) {
+    double mc = 0.0, mx = 0.0;
+    for (int k = 0; k < s; ++k) mc += aw[k] * A[(size_t)k * d + t];
+    for (int j = 0; j < m; ++j) mx += w[off + j] * supp[(size_t)(off + j) * d + t];
+    delta[t] = mc - mx;
+  }
+  __syncthreads();
+  // centre the pair: (a - mu_c) - (b - mu_x) = (a - delta) - b
+  for (int q = tid; q < s * d; q += LB_T) shA[q] -= delta[q % d];
+  double dd = 0.0;
+  for (int t = 0; t < d; ++t) dd += delta[t] * delta[t];
   __syncthreads();
 
   double colmin[LB_JPT];
@@ -156,7 +174,7 @@ __global__ __launch_bounds__(LB_T) void k_srv_bounds(int K, int s, int d,
     const double lc = red[0] + red[1] + red[2] + red[3];
     const double lr = red[4] + red[5] + red[6] + red[7];
     const size_t o = (size_t)r * K + c;
-    lb[o] = fmax(fmax(lc, lr), 0.0);
+    lb[o] = dd + fmax(fmax(lc, lr), 0.0);
     vals[o] = 0.0;
   }
 }
@@ -416,7 +434,7 @@ int launch_group(d2mi_server *S, SrvGroup &G) {
   D2MI_HIP(hipMemcpyAsync(base, h, L.in_end, hipMemcpyHostToDevice, st));
   k_srv_zero_status<<<1, 64, 0, st>>>((int *)(base + L.status),
                                       (unsigned long long *)(base + L.nlp));
-  const size_t lds = sizeof(double) * ((size_t)s * d + 4 * LB_KC);
+  const size_t lds = sizeof(double) * ((size_t)s * d + 4 * LB_KC + d);
   k_srv_bounds<<<R * K, LB_T, lds, st>>>(
       K, s, d, P.cs64.as<double>(), P.cw64.as<double>(), (const int *)(base + L.p_str),
       (const long long *)(base + L.cum), (const double *)(base + L.w),
@@ -564,7 +582,7 @@ extern "C" int d2mi_serve_open(d2mi_ctx *ctx, int phase, const d2mi_serve_opts *
     d2mi_set_error("the server needs resident Euclidean centroids on phase %d", phase);
     return D2MI_EINVAL;
   }
-  if (P->s > 64 * 8 || (size_t)P->s * P->dim * sizeof(double) > 48 * 1024) {
+  if (P->s > 64 * 8 || (size_t)(P->s + 1) * P->dim * sizeof(double) > 48 * 1024) {
     d2mi_set_error("centroids of %d x %d do not fit the bound kernel", P->s, P->dim);
     return D2MI_EUNSUPPORTED;
   }

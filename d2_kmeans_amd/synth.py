@@ -148,8 +148,10 @@ def serving_requests(n=1000, d=64, n_comp=64, m_lo=5, m_hi=500, mu=np.log(50.0),
     pts = means[comp] + 0.5 * rng.standard_normal((col, d))
     w = _dirichlet_ragged(rng, p_str)
     ph = Phase(metric=D2_EUCLIDEAN_L2, dim=d, p_str=p_str, p_w=w, p_supp=pts.reshape(-1), str=48)
+    # mixture component most of each query's points come from
+    dominant = dom[np.concatenate([[0], np.cumsum(p_str)[:-1]])] if n else dom
     return D2Data([ph]), dict(kind="serving_requests", n=n, d=d, m=(m_lo, m_hi), mu=float(mu),
-                              sigma=sigma, seed=seed)
+                              sigma=sigma, seed=seed, dominant=dominant)
 
 
 def seed_centroids(phase: Phase, K: int, s: int, labels=None, seed=SEED_BASE):

@@ -107,3 +107,45 @@ def test_word_embed_clustering_matches_oracle(engine):
     lab_o, it, _, cost = O.clustering(data, co, K, 6, opts, use_triangle=True)
     np.testing.assert_array_equal(engine.get_labels()[0], lab_o)
     assert rel(engine.get_centroids(0).p_supp, co[0].p_supp) < 1e-8
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_hbm_scratch_plan_matches_lds_plan(engine, precision):
+    """The plan kernel with its scratch in HBM (used when an object's plan
+    does not fit LDS), forced on a case the LDS form handles."""
+    say(f"HBM-scratch plan vs LDS plan, fp{precision}")
+    data, _ = synth.gaussian_mixture(n=800, seed=29)
+    ph = data.phases[0]
+    c0 = O.centroid_rands(ph, 6, 6, seed=0)
+    lab, lsw, _, _ = O.labeling(data, [c0])
+    opts = BADMMOptions(20, 2.0, 10)  # includes a residual pass (iteration 19)
+    sa, ca = device_update(engine, data, lab, lsw, c0, opts, precision)
+    os.environ["D2MI_BADMM_HBM_SCRATCH"] = "1"
+    try:
+        sb, cb = device_update(engine, data, lab, lsw, c0, opts, precision)
+    finally:
+        os.environ.pop("D2MI_BADMM_HBM_SCRATCH", None)
+    tol = 1e-12 if precision == 64 else 1e-5
+    assert rel(cb.p_w, ca.p_w) < tol
+    assert rel(cb.p_supp, ca.p_supp) < tol
+    assert abs(sb["obj"] - sa["obj"]) <= tol * abs(sa["obj"])
+    assert len(sb["trace"]) == 1 and len(sa["trace"]) == 1
+    np.testing.assert_allclose(sb["trace"], sa["trace"], rtol=1e-6)
+
+
+def test_large_objects_d64_match_oracle(engine):
+    """Objects of up to 500 points at d = 64 with s = 48: stored costs and the
+    HBM-scratch plan (the serving workload's training shape)."""
+    say("m up to 500, d=64, s=48 vs oracle")
+    data, _ = synth.serving_requests(n=160, d=64, m_lo=40, m_hi=500, seed=31)
+    ph = data.phases[0]
+    c0 = synth.seed_centroids(ph, 4, 48)
+    lab, lsw, _, _ = O.labeling(data, [c0])
+    assert len(np.unique(lab)) == 4
+    opts = BADMMOptions(10, 2.0, 10)
+    st64, c64 = device_update(engine, data, lab, lsw, c0, opts, 64)
+    co = c0.copy()
+    ro = O.badmm(ph, lab, np.ones(ph.size, np.int8), co, np.zeros(c0.s * ph.col), opts)
+    assert rel(c64.p_w, co.p_w) < 1e-9
+    assert rel(c64.p_supp, co.p_supp) < 1e-9
+    assert abs(st64["obj"] - ro["obj"]) <= 1e-9 * abs(ro["obj"])
