@@ -9,7 +9,8 @@
 //
 // Transportation simplex in FP64 on a spanning-tree basis (parent pointers,
 // root = row 0), run by all waves of the workgroup:
-//   * start: north-west corner;
+//   * start: Vogel's approximation (wave 0; about half the pivots of the
+//     north-west corner), rooted at row 0; north-west corner otherwise;
 //   * potentials, depths and the binary-lifting table lvl[b][v] (the 2^b-th
 //     ancestor of v) come out of one pointer-jumping pass per pivot: each
 //     round doubles the reach of every node, so a tree of depth D costs
@@ -227,7 +228,176 @@ struct Problem {
   const float *gt;        // TAB_GLOBAL32 source (global)
   float tab_rel;          // |table - exact| <= tab_rel * |table| + 1e-37
   int cpt;                // cells per thread per pricing block
+  int vogel;              // start from Vogel's approximation
 };
+
+// (value, index) order with the index breaking ties
+__device__ __forceinline__ bool lt_vi(float a, int ia, float b, int ib) {
+  return a < b || (a == b && ia < ib);
+}
+
+// two smallest (value, index) pairs; updated by value with selects only (a
+// conditional write through a reference becomes a private-memory store)
+struct Top2 {
+  float v1, v2;
+  int a1, a2;
+};
+
+__device__ __forceinline__ Top2 top2_insert(Top2 t, float c, int i) {
+  const bool b1 = lt_vi(c, i, t.v1, t.a1), b2 = lt_vi(c, i, t.v2, t.a2);
+  Top2 r;
+  r.v1 = b1 ? c : t.v1;
+  r.a1 = b1 ? i : t.a1;
+  r.v2 = b1 ? t.v1 : (b2 ? c : t.v2);
+  r.a2 = b1 ? t.a1 : (b2 ? i : t.a2);
+  return r;
+}
+
+// wave-wide merge, every lane gets the result
+__device__ __forceinline__ Top2 wave_top2(Top2 t) {
+  for (int off = 32; off > 0; off >>= 1) {
+    Top2 o;
+    o.v1 = __shfl_xor(t.v1, off, WAVE);
+    o.v2 = __shfl_xor(t.v2, off, WAVE);
+    o.a1 = __shfl_xor(t.a1, off, WAVE);
+    o.a2 = __shfl_xor(t.a2, off, WAVE);
+    t = top2_insert(top2_insert(t, o.v1, o.a1), o.v2, o.a2);
+  }
+  return t;
+}
+
+// Vogel's approximation, run by wave 0 alone (no workgroup barrier): each of
+// the n+m-1 steps takes the active line (row or column) whose two cheapest
+// active cells differ most (lowest line on ties), allocates its cheapest
+// cell as far as supply and demand allow and crosses out exactly one line,
+// so the cells form a spanning tree.  Line minima are kept per line and
+// recomputed only for lines whose two cheapest cells were crossed out.
+// Output (LDS arrays idle at this point): cell t = (depA[t], depB[t]) with
+// flow accB[t].  Scratch: accA = remaining supply / demand, flow = line
+// minima (as float pairs), lvl = their cells and the active flags.
+__device__ void vogel_start(const Problem &P, unsigned char *lds, const Layout &L,
+                            const float *ct, bool table) {
+  if (threadIdx.x >= WAVE) return;
+  const int lane = threadIdx.x;
+  const int n = P.n, m = P.m, nn = n + m;
+  double *rem = (double *)(lds + L.accA);
+  double *cq = (double *)(lds + L.accB);
+  int *ci = (int *)(lds + L.depA), *cj = (int *)(lds + L.depB);
+  float *mv1 = (float *)(lds + L.flow), *mv2 = mv1 + nn;
+  short *ma1 = (short *)(lds + L.lvl), *ma2 = ma1 + nn, *act = ma1 + 2 * nn;
+  auto cost = [&](int k, int j) -> float {
+    return table ? ct[k + j * n] : (float)P.C(k, j);
+  };
+  const Top2 none = {FLT_MAX, FLT_MAX, INT_MAX, INT_MAX};
+  auto store = [&](int v, const Top2 &t) {
+    mv1[v] = t.v1;
+    ma1[v] = (short)(t.a1 == INT_MAX ? -1 : t.a1);
+    mv2[v] = t.v2;
+    ma2[v] = (short)(t.a2 == INT_MAX ? -1 : t.a2);
+  };
+  // Scans load unconditionally (addresses stay in range) so the LDS reads
+  // of consecutive cells overlap instead of waiting on each other.
+  // row k over the active columns, by the whole wave
+  auto row_top2 = [&](int k) {
+    Top2 t = none;
+    for (int j = lane; j < m; j += WAVE) {
+      const float c = cost(k, j);
+      const bool on = act[n + j] != 0;
+      t = top2_insert(t, on ? c : FLT_MAX, on ? j : INT_MAX);
+    }
+    t = wave_top2(t);
+    if (lane == 0) store(k, t);
+  };
+  // column j over the active rows, by one lane
+  auto col_top2 = [&](int j) {
+    Top2 t = none;
+    int k = 0;
+    for (; k + 8 <= n; k += 8) {
+      float c[8];
+      bool on[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        c[u] = cost(k + u, j);
+        on[u] = act[k + u] != 0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) t = top2_insert(t, on[u] ? c[u] : FLT_MAX, on[u] ? k + u : INT_MAX);
+    }
+    for (; k < n; ++k) {
+      const float c = cost(k, j);
+      const bool on = act[k] != 0;
+      t = top2_insert(t, on ? c : FLT_MAX, on ? k : INT_MAX);
+    }
+    store(n + j, t);
+  };
+  // supplies and demands; the last column takes the remainder (its
+  // constraint is the one the LP drops)
+  double sbm = 0.0, sa = 0.0;
+  for (int k = lane; k < n; k += WAVE) { rem[k] = P.wa[k]; sa += P.wa[k]; }
+  for (int j = lane; j < m - 1; j += WAVE) { rem[n + j] = P.wb[j]; sbm += P.wb[j]; }
+  sa = wave_sum_d(sa);
+  sbm = wave_sum_d(sbm);
+  if (lane == 0) rem[n + m - 1] = sa - sbm > 0.0 ? sa - sbm : 0.0;
+  for (int v = lane; v < nn; v += WAVE) act[v] = 1;
+  __builtin_amdgcn_wave_barrier();
+  for (int k = 0; k < n; ++k) row_top2(k);
+  for (int j = lane; j < m; j += WAVE) col_top2(j);
+  __builtin_amdgcn_wave_barrier();
+  int nr = n, nc = m;
+  for (int t = 0; t < nn - 1; ++t) {
+    // line with the largest penalty
+    float bp = -1.0f;
+    int bl = INT_MAX;
+    for (int v = lane; v < nn; v += WAVE) {
+      const bool on = act[v] != 0;
+      const float p2 = mv2[v], p1 = mv1[v];
+      const float pen = !on ? -2.0f : (ma2[v] >= 0 ? p2 - p1 : 0.0f);
+      const bool better = pen > bp || (pen == bp && v < bl);
+      bp = better ? pen : bp;
+      bl = better ? v : bl;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      const float op = __shfl_xor(bp, off, WAVE);
+      const int ol = __shfl_xor(bl, off, WAVE);
+      if (op > bp || (op == bp && ol < bl)) { bp = op; bl = ol; }
+    }
+    const int is = bl < n ? bl : ma1[bl];
+    const int js = bl < n ? ma1[bl] : bl - n;
+    const double ra = rem[is], rb = rem[n + js];
+    double q = fmin(ra, rb);
+    if (q < 0.0) q = 0.0;
+    const bool cross_row = (ra - q <= rb - q && nr > 1) || nc == 1;
+    __builtin_amdgcn_wave_barrier();
+    if (lane == 0) {
+      rem[is] = ra - q;
+      rem[n + js] = rb - q;
+      ci[t] = is;
+      cj[t] = js;
+      cq[t] = q;
+      act[cross_row ? is : n + js] = 0;
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (t == nn - 2) break;
+    if (cross_row) {
+      --nr;
+      for (int j = lane; j < m; j += WAVE)
+        if (act[n + j] && (ma1[n + j] == is || ma2[n + j] == is)) col_top2(j);
+    } else {
+      --nc;
+      for (int b = 0; b < n; b += WAVE) {
+        const int k = b + lane;
+        const bool aff = k < n && act[k] && (ma1[k] == js || ma2[k] == js);
+        unsigned long long mask = __ballot(aff);
+        while (mask) {
+          const int bit = __ffsll((long long)mask) - 1;
+          row_top2(b + bit);
+          mask &= mask - 1;
+        }
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+}
 
 // Solves one LP with the whole workgroup (blockDim.x a multiple of 64, at
 // most 64*MAX_WAVES threads); returns the optimal value to all threads.
@@ -280,31 +450,60 @@ __device__ double solve(const Problem &P, unsigned char *lds, const Layout &L, i
   cmax = block_max(cmax, ctl, bank, nw);
   const double eps = 1e-12 * (1.0 + cmax);
 
-  // ---- north-west corner: each new cell brings one new node, hung under
-  // the endpoint already in the tree; flow[v] = flow on v's parent edge
-  if (tid == 0) {
-    double sa = 0.0, sbm = 0.0;
-    for (int k = 0; k < n; ++k) { accA[k] = P.wa[k]; sa += P.wa[k]; }
-    for (int j = 0; j < m - 1; ++j) { accB[j] = P.wb[j]; sbm += P.wb[j]; }
-    accB[m - 1] = sa - sbm > 0.0 ? sa - sbm : 0.0;
-    par[0] = -1;
-    flow[0] = 0.0;
-    int i = 0, j = 0, newnode = n;  // the first cell (0,0) brings column 0
-    for (int t = 0; t < nn - 1; ++t) {
-      double q = fmin(accA[i], accB[j]);
-      if (q < 0.0) q = 0.0;
-      par[newnode] = newnode >= n ? i : n + j;
-      flow[newnode] = q;
-      accA[i] -= q;
-      accB[j] -= q;
-      if (t == nn - 2) break;
-      if (i == n - 1) { ++j; newnode = n + j; }
-      else if (j == m - 1) { ++i; newnode = i; }
-      else if (accA[i] <= accB[j]) { ++i; newnode = i; }
-      else { ++j; newnode = n + j; }
+  // ---- starting basis: Vogel's approximation (about half the pivots of the
+  // north-west corner), else the north-west corner
+  bool have_start = false;
+  if (P.vogel && n > 1 && m > 1) {
+    vogel_start(P, lds, L, ct, table);
+    __syncthreads();
+    // root the basis cells at row 0: a node whose tree neighbour is already
+    // rooted hangs under it (each node is reached exactly once in a tree)
+    const int *ci = (const int *)(lds + L.depA), *cj = (const int *)(lds + L.depB);
+    const double *cq = (const double *)(lds + L.accB);
+    for (int v = tid; v < nn; v += T) par[v] = v == 0 ? -1 : -2;
+    if (tid == 0) flow[0] = 0.0;
+    __syncthreads();
+    for (int round = 0; round < nn; ++round) {
+      int changed = 0;
+      for (int t = tid; t < nn - 1; t += T) {
+        const int i = ci[t], v = n + cj[t];
+        const int pi = par[i], pv = par[v];
+        if (pi != -2 && pv == -2) { par[v] = i; flow[v] = cq[t]; changed = 1; }
+        else if (pv != -2 && pi == -2) { par[i] = v; flow[i] = cq[t]; changed = 1; }
+      }
+      if (!block_or(changed, ctl, bank, nw)) break;
     }
+    int missing = 0;
+    for (int v = tid; v < nn; v += T) missing |= par[v] == -2;
+    have_start = !block_or(missing, ctl, bank, nw);
   }
-  __syncthreads();
+  if (!have_start) {
+    // north-west corner: each new cell brings one new node, hung under the
+    // endpoint already in the tree; flow[v] = flow on v's parent edge
+    if (tid == 0) {
+      double sa = 0.0, sbm = 0.0;
+      for (int k = 0; k < n; ++k) { accA[k] = P.wa[k]; sa += P.wa[k]; }
+      for (int j = 0; j < m - 1; ++j) { accB[j] = P.wb[j]; sbm += P.wb[j]; }
+      accB[m - 1] = sa - sbm > 0.0 ? sa - sbm : 0.0;
+      par[0] = -1;
+      flow[0] = 0.0;
+      int i = 0, j = 0, newnode = n;  // the first cell (0,0) brings column 0
+      for (int t = 0; t < nn - 1; ++t) {
+        double q = fmin(accA[i], accB[j]);
+        if (q < 0.0) q = 0.0;
+        par[newnode] = newnode >= n ? i : n + j;
+        flow[newnode] = q;
+        accA[i] -= q;
+        accB[j] -= q;
+        if (t == nn - 2) break;
+        if (i == n - 1) { ++j; newnode = n + j; }
+        else if (j == m - 1) { ++i; newnode = i; }
+        else if (accA[i] <= accB[j]) { ++i; newnode = i; }
+        else { ++j; newnode = n + j; }
+      }
+    }
+    __syncthreads();
+  }
 
   // cost of the tree edge at node v (edge to parent p), table or exact
   auto edge_cell = [&](int v, int p) { return v < n ? v + (p - n) * n : p + (v - n) * n; };

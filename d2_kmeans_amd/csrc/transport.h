@@ -42,7 +42,7 @@ struct LpArgs {
   long long gstride;
   int n_max, m_max;
   // launch shape, filled by d2mi_plan_transport
-  int threads, cpt, table, stage_a, stage_b;
+  int threads, cpt, table, stage_a, stage_b, vogel;
   size_t lds;
   int threads_hint;     // > 0: workgroup size to use
   hipStream_t stream;   // null: the context stream

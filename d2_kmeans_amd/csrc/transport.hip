@@ -82,6 +82,7 @@ __global__ __launch_bounds__(1024) void k_transport(LpArgs a) {
     P.gt = nullptr;
     P.tab_rel = 0x1p-22f;
     P.cpt = a.cpt;
+    P.vogel = a.vogel;
     int piv = 0;
     const double v = lp::solve(P, lds, L, &piv, a.status);
     if (tid == 0) {
@@ -153,6 +154,7 @@ int d2mi_plan_transport(d2mi_ctx *ctx, LpArgs &a) {
   T = std::max(64, std::min(1024, T / 64 * 64));
   a.threads = T;
   a.cpt = std::max(1, std::min(lp::CPT_MAX, env_int("D2MI_LP_BLOCK", 2048) / T));
+  a.vogel = env_int("D2MI_LP_VOGEL", 1) != 0;
   return D2MI_OK;
 }
 
