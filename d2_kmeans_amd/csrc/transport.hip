@@ -154,7 +154,10 @@ int d2mi_plan_transport(d2mi_ctx *ctx, LpArgs &a) {
   T = std::max(64, std::min(1024, T / 64 * 64));
   a.threads = T;
   a.cpt = std::max(1, std::min(lp::CPT_MAX, env_int("D2MI_LP_BLOCK", 2048) / T));
-  a.vogel = env_int("D2MI_LP_VOGEL", 1) != 0;
+  // histogram ground costs on ordered bins (|i-j|, (i-j)^2, ...) usually form
+  // a Monge matrix, where the north-west corner is already optimal
+  const bool hist = a.cost_mode == LP_COST_HIST || a.cost_mode == LP_COST_SPARSE;
+  a.vogel = env_int("D2MI_LP_VOGEL", hist ? 0 : 1) != 0;
   return D2MI_OK;
 }
 

@@ -18,12 +18,17 @@ Seeds default to 20261203 + workload index.
 * :func:`serving_requests` -- request distributions for assignment: sizes
   lognormal(ln 50, 1) clipped to [5,500], Dirichlet(1) weights, points from
   a d-dim Gaussian mixture.
+* :func:`multiphase_histograms` -- three phases per object: Euclidean d=3
+  with 8-32 Gaussian-mixture points; a dense 64-bin histogram with ground
+  cost (i-j)^2; a sparse 1024-bin histogram with 5-40 nonzeros, ground cost
+  |i-j| and Dirichlet(0.5) weights.  K=32.
 """
 from __future__ import annotations
 
 import numpy as np
 
-from .data import D2_EUCLIDEAN_L2, D2_WORD_EMBED, D2Data, Phase
+from .data import (D2_EUCLIDEAN_L2, D2_HISTOGRAM, D2_SPARSE_HISTOGRAM, D2_WORD_EMBED, D2Data,
+                   Phase)
 
 SEED_BASE = 20261203
 
@@ -177,3 +182,49 @@ def seed_centroids(phase: Phase, K: int, s: int, labels=None, seed=SEED_BASE):
         if c.p_supp is not None:
             c.p_supp[k * s * phase.dim:(k + 1) * s * phase.dim] = sup[top].reshape(-1)
     return c
+
+
+def multiphase_histograms(n=100_000, V1=64, V2=1024, n_comp=32, seed=SEED_BASE + 3):
+    """Co-clustering workload with three phases.  Each object draws a
+    component; its points, its dense histogram (a discretised Gaussian bump
+    around a component centre) and its sparse histogram's bins (around
+    another centre) follow that component."""
+    rng = np.random.default_rng(seed)
+    comp = rng.integers(0, n_comp, size=n)
+    # phase 0: Euclidean d=3
+    means = rng.uniform(-5, 5, size=(n_comp, 3))
+    m0 = rng.integers(8, 33, size=n).astype(np.int32)
+    c0 = np.repeat(comp, m0)
+    pts = means[c0] + 0.5 * rng.standard_normal((len(c0), 3))
+    w0 = _dirichlet_ragged(rng, m0)
+    ph0 = Phase(metric=D2_EUCLIDEAN_L2, dim=3, p_str=m0, p_w=w0, p_supp=pts.reshape(-1), str=16)
+    # phase 1: dense V1-bin histograms, cost (i-j)^2
+    ctr1 = rng.uniform(8, V1 - 8, size=n_comp)
+    bins = np.arange(V1)
+    centre = ctr1[comp] + rng.normal(0, 2.0, size=n)
+    width = rng.uniform(3.0, 8.0, size=n)
+    h = np.exp(-0.5 * ((bins[None, :] - centre[:, None]) / width[:, None]) ** 2) + 1e-3
+    h /= h.sum(1, keepdims=True)
+    d1 = (bins[:, None] - bins[None, :]).astype(np.float64) ** 2
+    ph1 = Phase(metric=D2_HISTOGRAM, dim=0, p_str=np.full(n, V1, np.int32), p_w=h.reshape(-1),
+                vocab_size=V1, dist_mat=d1.reshape(-1), str=V1)
+    # phase 2: sparse V2-bin histograms, 5-40 nonzeros, cost |i-j|, Dirichlet(0.5)
+    ctr2 = rng.uniform(64, V2 - 64, size=n_comp)
+    m2 = rng.integers(5, 41, size=n).astype(np.int32)
+    sym = []
+    for i in range(n):
+        cand = np.unique(np.clip(np.round(rng.normal(ctr2[comp[i]], 40.0, size=3 * m2[i])), 0,
+                                 V2 - 1).astype(np.int64))
+        while len(cand) < m2[i]:
+            cand = np.unique(np.concatenate([cand, rng.integers(0, V2, m2[i])]))
+        sym.append(np.sort(rng.choice(cand, size=m2[i], replace=False)))
+    sym = np.concatenate(sym).astype(np.int32)
+    w2 = np.maximum(rng.gamma(0.5, 1.0, size=len(sym)), 1e-6)
+    starts = np.concatenate([[0], np.cumsum(m2)[:-1]])
+    w2 = w2 / np.repeat(np.add.reduceat(w2, starts), m2)
+    b2 = np.arange(V2)
+    d2 = np.abs(b2[:, None] - b2[None, :]).astype(np.float64)
+    ph2 = Phase(metric=D2_SPARSE_HISTOGRAM, dim=0, p_str=m2, p_w=w2, p_supp_sym=sym,
+                vocab_size=V2, dist_mat=d2.reshape(-1), str=V2)
+    return D2Data([ph0, ph1, ph2]), dict(kind="multiphase_histograms", n=n, V1=V1, V2=V2,
+                                         n_comp=n_comp, seed=seed, classes=comp.astype(np.int32))

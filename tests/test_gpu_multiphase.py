@@ -1,0 +1,100 @@
+"""Multi-phase co-clustering with histogram phases (Euclidean d=3, a dense
+64-bin histogram with cost (i-j)^2, a sparse 1024-bin histogram with cost
+|i-j| and centroids of s = V = 1024 bins, as the reference forces for sparse
+histograms, clustering_io.c:182-185), against the FP64 oracle.  The sparse
+phase's plans (1024 x m) exceed LDS, so its B-ADMM runs the HBM-scratch plan
+kernel."""
+import numpy as np
+import pytest
+
+from d2_kmeans_amd import synth
+from d2_kmeans_amd.data import BADMMOptions, Centroids, D2Data
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(1e-300, np.max(np.abs(b))))
+
+
+@pytest.fixture(scope="module")
+def c4():
+    data, meta = synth.multiphase_histograms(n=120, seed=91)
+    return data, meta["classes"]
+
+
+def hist_centroids(ph, K, seed):
+    rng = np.random.default_rng(seed)
+    V = ph.vocab_size
+    return Centroids(K, V, 0, rng.dirichlet(np.ones(V), size=K).reshape(-1))
+
+
+@pytest.mark.parametrize("which", [1, 2])
+def test_histogram_phase_badmm_vs_oracle(engine, c4, which):
+    data, cls = c4
+    ph = data.phases[which]
+    say(f"histogram phase {which}: V={ph.vocab_size}, m_max={ph.p_str.max()}")
+    K = 4
+    c0 = hist_centroids(ph, K, which)
+    lab = (cls % K).astype(np.int32)
+    lsw = np.ones(ph.size, np.int8)
+    one = D2Data([ph])
+    opts = BADMMOptions(10, 2.0, 10)
+    co = c0.copy()
+    ro = O.badmm(ph, lab, lsw, co, np.zeros(c0.s * ph.col), opts)
+    for precision in (64, 32):
+        engine.set_data(one)
+        engine.set_centroids(0, c0)
+        engine.set_labels(lab, lsw)
+        st = engine.badmm(0, opts, precision=precision)
+        cg = engine.get_centroids(0)
+        tol = 1e-9 if precision == 64 else 1e-4
+        assert rel(cg.p_w, co.p_w) < tol
+        assert abs(st["obj"] - ro["obj"]) <= (1e-9 if precision == 64 else 1e-5) * abs(ro["obj"])
+
+
+def test_histogram_distances_vs_oracle(engine, c4):
+    """Exact LP distances over all three phases summed (d2_compute_distance,
+    clustering.c:33-128)."""
+    data, cls = c4
+    say("multi-phase distances")
+    K = 3
+    cents = [O.centroid_rands(data.phases[0], K, 16, seed=0)] + \
+            [hist_centroids(data.phases[p], K, 10 + p) for p in (1, 2)]
+    engine.set_data(data)
+    for p, c in enumerate(cents):
+        engine.set_centroids(p, c)
+    engine.set_labels(np.zeros(data.size, np.int32), np.zeros(data.size, np.int8))
+    obj = np.repeat(np.arange(30), K).astype(np.int32)
+    cen = np.tile(np.arange(K), 30).astype(np.int32)
+    got = engine.distances(obj, cen)
+    ref = np.array([O.distance(data, int(i), cents, int(j)) for i, j in zip(obj, cen)])
+    np.testing.assert_allclose(got, ref, rtol=1e-11, atol=1e-12)
+    for p in range(3):
+        got_p = engine.distances(obj[:12], cen[:12], selected_phase=p)
+        ref_p = np.array([O.distance(data, int(i), cents, int(j), selected_phase=p)
+                          for i, j in zip(obj[:12], cen[:12])])
+        np.testing.assert_allclose(got_p, ref_p, rtol=1e-11, atol=1e-12)
+
+
+def test_multiphase_clustering_vs_oracle(engine, c4):
+    data, cls = c4
+    say("multi-phase clustering loop")
+    K = 4
+    cents = [O.centroid_rands(data.phases[0], K, 16, seed=0)] + \
+            [hist_centroids(data.phases[p], K, 20 + p) for p in (1, 2)]
+    engine.set_data(data)
+    for p, c in enumerate(cents):
+        engine.set_centroids(p, c)
+    opts = BADMMOptions(10, 2.0, 10)
+    res = engine.clustering(K, max_iter=3, opts=opts, use_triangle=False, precision=64)
+    co = [c.copy() for c in cents]
+    lab_o, it, _, cost = O.clustering(data, co, K, 3, opts, use_triangle=False)
+    np.testing.assert_array_equal(engine.get_labels()[0], lab_o)
+    for p in range(3):
+        assert rel(engine.get_centroids(p).p_w, co[p].p_w) < 1e-8
