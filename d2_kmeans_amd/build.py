@@ -10,8 +10,8 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libd2mi.so")
-SOURCES = ["context.hip", "badmm.hip", "transport.hip", "labeling.hip", "serve.hip",
-           "compat.hip"]
+SOURCES = ["context.hip", "badmm.hip", "pdist2.hip", "transport.hip", "labeling.hip",
+           "serve.hip", "compat.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("D2MI_ARCH", "gfx950")
 

@@ -25,6 +25,9 @@
 
 #include "d2mi_internal.h"
 
+int d2mi_pdist2_mfma_dev(hipStream_t S, int P, int Q, int d, const float *x, const float *y,
+                         float *nx, float *ny, float *T, int bf16);
+
 namespace {
 
 constexpr int NT = 256;        // threads per workgroup (4 waves of 64)
@@ -540,6 +543,28 @@ struct TabArgs {
 
 constexpr int TAB_KC = 16;  // centroid rows per thread
 
+// Word embeddings in FP32: C of object i, element (k, j) = W[id_j * (K s) +
+// label_i * s + k], where W = pdist2(vocab, all centroid supports) comes from
+// the matrix cores (pdist2.hip); a thread per element, 64 consecutive k of
+// one word are one contiguous read.
+__global__ __launch_bounds__(NT) void k_word_gather(int N, int s, int Ks,
+                                                    const int *__restrict__ p_str,
+                                                    const long long *__restrict__ cum,
+                                                    const int *__restrict__ label,
+                                                    const int *__restrict__ sym,
+                                                    const float *__restrict__ W,
+                                                    float *__restrict__ ctab) {
+  for (int o = blockIdx.x; o < N; o += gridDim.x) {
+    const int m = p_str[o], l = label[o];
+    const long long off = cum[o];
+    float *C = ctab + (size_t)s * off;
+    for (int e = threadIdx.x; e < s * m; e += NT) {
+      const int j = e / s, k = e - (e / s) * s;
+      C[e] = W[(size_t)sym[off + j] * Ks + (size_t)l * s + k];
+    }
+  }
+}
+
 // calculate_distmat (centroid_util.c:8-36): _dpdist2 for Euclidean phases,
 // _dpdist2_sym (blas_like64.c:210-220) for word embeddings, by direct
 // differences.  A workgroup walks a chunk of the label-sorted order with its
@@ -1048,7 +1073,23 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
   // COST_TABLE: rebuild C from the current centroid supports (calculate_distmat)
   const int tab_G = std::max(1, std::min(N, ctx->num_cu * 4));
   const int tab_chunk = (N + tab_G - 1) / tab_G;
-  auto build_table = [&](hipStream_t S) {
+  const bool mfma_words = word && !F64 && !getenv_flag("D2MI_WORD_DIRECT");
+  if (CM == COST_TABLE && mfma_words) {
+    D2MI_TRY(ctx->wtab.ensure(sizeof(float) * (size_t)p->V * K * s));
+    D2MI_TRY(ctx->wnorm.ensure(sizeof(float) * ((size_t)p->V + (size_t)K * s)));
+  }
+  auto build_table = [&](hipStream_t S) -> int {
+    if (mfma_words) {  // one GEMM for every (word, centroid point) pair, then a gather
+      float *nv = ctx->wnorm.as<float>(), *nc = nv + p->V;
+      D2MI_TRY(d2mi_pdist2_mfma_dev(S, p->V, K * s, dsup, (const float *)p->vocab32.p,
+                                    (const float *)csT, nv, nc, ctx->wtab.as<float>(),
+                                    getenv_flag("D2MI_PDIST2_BF16") ? 1 : 0));
+      k_word_gather<<<std::min(N, ctx->num_cu * 16), NT, 0, S>>>(
+          N, s, K * s, p->p_str.as<int>(), p->cum.as<long long>(), ctx->label.as<int>(), sym,
+          ctx->wtab.as<float>(), (float *)ctab);
+      D2MI_HIP(hipGetLastError());
+      return D2MI_OK;
+    }
     TabArgs<T> ta{};
     ta.N = N; ta.s = s; ta.d = dsup; ta.chunk = tab_chunk; ta.perm = ctx->perm.as<int>();
     ta.p_str = p->p_str.as<int>(); ta.cum = p->cum.as<long long>(); ta.label = ctx->label.as<int>();
@@ -1057,6 +1098,8 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
     const int G = (N + tab_chunk - 1) / tab_chunk;
     if (word) k_badmm_cost_table<T, true><<<G, NT, lds, S>>>(ta);
     else k_badmm_cost_table<T, false><<<G, NT, lds, S>>>(ta);
+    D2MI_HIP(hipGetLastError());
+    return D2MI_OK;
   };
   // COST_TABLE support update: numerators from X, then supports
   auto update_supports = [&](hipStream_t S, int *seg, int *flags) -> int {
@@ -1110,7 +1153,7 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
 
   // ---- 2. setup
   D2MI_HIP(hipEventRecord(ctx->ev[0], S));
-  if constexpr (CM == COST_TABLE) build_table(S);
+  if constexpr (CM == COST_TABLE) D2MI_TRY(build_table(S));
   const int setup_per_block = std::max(1, N / (ctx->num_cu * 8));
   const int setup_blocks = (N + setup_per_block - 1) / setup_per_block;
   D2MI_TRY(ctx->part.ensure(sizeof(double) * (size_t)std::max(setup_blocks, 3 * 65536)));
@@ -1228,7 +1271,7 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
       D2MI_HIP(hipGetLastError());
       if (CM == COST_TABLE && upd) {  // step 5 (:196-227): supports, then C again
         D2MI_TRY(update_supports(S, seg, flags));
-        build_table(S);
+        D2MI_TRY(build_table(S));
       }
       ++n_launch;
     }
@@ -1257,7 +1300,7 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
                                                  p->cw64.as<double>(), cwT,
                                                  p->cs64.as<double>(), csT, flags);
       D2MI_HIP(hipGetLastError());
-      if (CM == COST_TABLE && upd) build_table(S);
+      if (CM == COST_TABLE && upd) D2MI_TRY(build_table(S));
       ++n_launch;
     }
   }

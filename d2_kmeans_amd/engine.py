@@ -136,6 +136,19 @@ class Engine:
         check(lib().d2mi_pdist2(self._h, d, n, m, ptr(A), ptr(B), ptr(out)))
         return out
 
+    def pdist2_mfma(self, A, B, bf16: bool = False, timing: bool = False):
+        """The same table as :meth:`pdist2` from FP32 inputs on the matrix
+        cores (FP32 or BF16 products, FP32 norms and accumulation)."""
+        A = np.ascontiguousarray(A, np.float64)
+        B = np.ascontiguousarray(B, np.float64)
+        n, d = A.shape
+        m = B.shape[0]
+        out = np.empty(n * m)
+        ms = C.c_float(0)
+        check(lib().d2mi_pdist2_mfma(self._h, d, n, m, ptr(A), ptr(B), ptr(out), int(bf16),
+                                     C.byref(ms)))
+        return (out, ms.value) if timing else out
+
     # ------------------------------------------------------- transport LP
     def transport_batch(self, C, wX, wY, pivots=False):
         """Exact LP values of a batch of same-shape transport problems

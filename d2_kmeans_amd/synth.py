@@ -123,10 +123,11 @@ def word_embed_docs(n=20_000, V=50_000, d=300, m_lo=10, m_hi=200, seed=SEED_BASE
     vocab /= np.linalg.norm(vocab, axis=1, keepdims=True)
     p_str = _zipf_trunc(rng, 1.1, m_lo, m_hi, n).astype(np.int32)
     rankp = 1.0 / np.arange(1, V + 1)
-    rankp /= rankp.sum()
+    cdf = np.cumsum(rankp / rankp.sum())
+    cdf[-1] = 1.0
     ids = []
-    for m in p_str:  # distinct word ids per document
-        ids.append(np.unique(rng.choice(V, size=int(m) * 2, p=rankp))[:m])
+    for m in p_str:  # distinct word ids per document, Zipf over the vocabulary
+        ids.append(np.unique(np.searchsorted(cdf, rng.random(int(m) * 2), side="right"))[:m])
     # np.unique may return fewer than m ids: top up with uniform draws
     for t, m in enumerate(p_str):
         while len(ids[t]) < m:
@@ -177,6 +178,8 @@ def seed_centroids(phase: Phase, K: int, s: int, labels=None, seed=SEED_BASE):
         if len(cand) == 0:
             raise ValueError(f"no object with >= {s} supports for cluster {k}")
         w, sup = phase.object(int(cand[0]))
+        if phase.metric == D2_WORD_EMBED:  # supports are the words' embeddings
+            sup = phase.vocab_vec.reshape(phase.vocab_size, phase.dim)[sup]
         top = np.sort(np.argsort(-w, kind="stable")[:s])
         c.p_w[k * s:(k + 1) * s] = w[top] / w[top].sum()
         if c.p_supp is not None:
