@@ -21,7 +21,10 @@ resident centroids of 48 points in d=64; query sizes lognormal(ln 50, 1)
 clipped to [5, 500], Dirichlet(1) weights, Gaussian-mixture supports;
 Poisson arrivals at a fixed rate, served with continuous batching, with
 p50/p99 latency and queries/s beside the one-query-per-launch form on the
-same arrivals.  Prints ONE JSON line on stdout; progress goes to stderr.
+same arrivals.  The ``word_embed`` object covers the word-embedding
+workload (V=50k, d=300): FP32 B-ADMM with MFMA cost tables gated against
+FP64, and the pdist2 GEMM against the FP32 matrix peak.  Prints ONE JSON line
+on stdout; progress goes to stderr.
 """
 from __future__ import annotations
 
@@ -189,6 +192,58 @@ def assign_leg(eng, args):
     return res
 
 
+def word_embed_leg(eng):
+    """Word-embedding D2 (V=50k, d=300, N=20k documents, K=20, s=64): one
+    B-ADMM update in FP32 with the cost table from the matrix cores, gated
+    against an FP64 update from the same state; pdist2 GEMM throughput in
+    FP32 and BF16 against the FP32 matrix peak."""
+    from d2_kmeans_amd import synth
+    from d2_kmeans_amd.data import BADMMOptions
+    data, meta = synth.word_embed_docs()
+    ph = data.phases[0]
+    K, s = 20, 64
+    c0 = synth.seed_centroids(ph, K, s)
+    N = ph.size
+    labels = (np.arange(N) % K).astype(np.int32)  # balanced, deterministic
+    eng.set_data(data)
+    opts = BADMMOptions(10, 2.0, 10)
+    out = {"workload": f"word-embedding D2: V={ph.vocab_size}, d={ph.dim}, N={N} documents "
+                       f"(Zipf lengths, mean {ph.p_str.mean():.1f}), K={K}, s={s}, labels i mod K",
+           "plan_elements": int(s * ph.col)}
+    objs = {}
+    for name, prec, env in (("fp64", 64, {}), ("fp32_mfma", 32, {}),
+                            ("fp32_bf16", 32, {"D2MI_PDIST2_BF16": "1"})):
+        os.environ.update(env)
+        eng.set_centroids(0, c0)
+        eng.set_labels(labels, np.ones(N, np.int8))
+        eng.badmm(0, opts, precision=prec)  # warm (plans allocated)
+        eng.set_centroids(0, c0)
+        eng.set_labels(labels, np.zeros(N, np.int8))
+        t0 = time.perf_counter()
+        st = eng.badmm(0, opts, precision=prec)
+        dt = time.perf_counter() - t0
+        for k in env:
+            os.environ.pop(k)
+        objs[name] = st["obj"]
+        out[name] = {"plan_updates_per_s": N * (opts.maxIters + 1) / dt, "ms_per_update": 1e3 * dt,
+                     "plan_kernel_ms_total": st["ms_plan"], "obj": st["obj"]}
+    for name in ("fp32_mfma", "fp32_bf16"):
+        out[name]["obj_rel_err_vs_fp64"] = abs(objs[name] - objs["fp64"]) / abs(objs["fp64"])
+    V, d = ph.vocab_size, ph.dim
+    vocab = ph.vocab_vec.reshape(V, d)
+    cs = c0.p_supp.reshape(K * s, d)
+    flops = 2.0 * V * K * s * d
+    for bf in (False, True):
+        eng.pdist2_mfma(cs[:8], vocab[:64], bf16=bf)
+        _, ms = eng.pdist2_mfma(cs, vocab, bf16=bf, timing=True)
+        peak = 2500.0 if bf else 157.3
+        out["pdist2_bf16" if bf else "pdist2_fp32"] = {
+            "shape": f"{V} x {K * s} x {d}", "ms": ms, "tflops": flops / ms / 1e9,
+            "peak_tflops": peak, "frac": flops / ms / 1e9 / peak}
+    log(f"[word-embed] {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -200,6 +255,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-gate", action="store_true")
     ap.add_argument("--no-assign", action="store_true")
+    ap.add_argument("--no-word-embed", action="store_true")
     ap.add_argument("--assign-n", type=int, default=3000)
     ap.add_argument("--assign-qps", type=float, default=1000.0)
     ap.add_argument("--assign-plain-n", type=int, default=300)
@@ -303,6 +359,7 @@ def main():
         log(f"[bench] cpu baseline: {cpu}")
 
     assign = None if args.no_assign else assign_leg(eng, args)
+    word = None if args.no_word_embed else word_embed_leg(eng)
 
     out = {
         "metric": METRIC, "value": value, "unit": "plan updates/s", "n_gpus": 1,
@@ -323,6 +380,7 @@ def main():
         "speedup_vs_plain": None if plain is None else value / plain["value"],
         "cpu_baseline": cpu,
         "assign": assign,
+        "word_embed": word,
     }
     print(json.dumps(out), flush=True)
     eng.close()
