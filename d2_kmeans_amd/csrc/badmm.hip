@@ -630,56 +630,78 @@ struct SuppArgs {
 };
 
 constexpr int SUP_KPT = 16;  // rows per thread: s <= 4 * SUP_KPT
+constexpr int SUP_TT = 128;  // dimensions per tile (two per lane)
 
 // Step 5 numerators: sum over the label's objects of supp_i X_i^T
 // (centroid_Bregman.c:199-212, one dgemm per object) or of X_i[k, j] *
 // vocab[id_j] (:229-247).  Records per (workgroup, label) at g + l as in the
-// plan kernel; a thread owns dimension t = t0 + lane and rows k = wave + 4 i,
-// with FP64 accumulators.
+// plan kernel.  Per dimension tile, each object's X and its supports (or
+// word vectors) are staged through LDS in blocks of columns (128 in FP32,
+// 64 in FP64), all loads of a block in flight at once; a thread owns
+// dimensions t0 + lane and t0 + 64 + lane and rows k = wave + 4 i, with FP64
+// accumulators.
 template <typename T, bool WORD>
 __global__ __launch_bounds__(NT) void k_badmm_supp_num(SuppArgs<T> a) {
+  constexpr int SUP_JB = sizeof(T) == 8 ? 64 : 128;
+  __shared__ T Xl[4 * SUP_KPT * SUP_JB];  // [k][j] of the block
+  __shared__ T Vl[SUP_JB * SUP_TT];       // [j][t] of the block
   const int s = a.s, d = a.d, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int g = blockIdx.x;
   const int p0 = g * a.chunk, p1 = min(a.N, p0 + a.chunk);
   const size_t R = (size_t)s * d;
-  for (int t0 = 0; t0 < d; t0 += 64) {
-    const int t = t0 + lane;
-    double acc[SUP_KPT];
+  for (int t0 = 0; t0 < d; t0 += SUP_TT) {
+    double acc[SUP_KPT][2];
     int cur = -1;
+    auto flush = [&](int l) {
+#pragma unroll
+      for (int i = 0; i < SUP_KPT; ++i) {
+        const int k = wv + 4 * i;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int t = t0 + h * 64 + lane;
+          if (k < s && t < d) a.rec[(size_t)(g + l) * R + (size_t)k * d + t] = acc[i][h];
+        }
+      }
+    };
     for (int p = p0; p < p1; ++p) {
       const int o = a.perm[p], l = a.label[o], m = a.p_str[o];
       const long long off = a.cum[o];
       if (l != cur) {
-        if (cur >= 0) {
+        if (cur >= 0) flush(cur);
 #pragma unroll
-          for (int i = 0; i < SUP_KPT; ++i) {
-            const int k = wv + 4 * i;
-            if (k < s && t < d) a.rec[(size_t)(g + cur) * R + (size_t)k * d + t] = acc[i];
-          }
-        }
-#pragma unroll
-        for (int i = 0; i < SUP_KPT; ++i) acc[i] = 0.0;
+        for (int i = 0; i < SUP_KPT; ++i) acc[i][0] = acc[i][1] = 0.0;
         cur = l;
       }
       const T *Xo = a.X + (size_t)s * off;
-      for (int j = 0; j < m; ++j) {
-        const T *x = WORD ? a.vocab + (size_t)a.sym[off + j] * d : a.supp + (size_t)(off + j) * d;
-        const double xv = t < d ? (double)x[t] : 0.0;
-        const T *xc = Xo + (size_t)j * s;
+      for (int j0 = 0; j0 < m; j0 += SUP_JB) {
+        const int jn = min(SUP_JB, m - j0);
+        __syncthreads();  // the previous block is consumed
+        for (int e = tid; e < s * jn; e += NT) {  // X block, column-major in HBM
+          const int j = e / s, k = e - (e / s) * s;
+          Xl[k * SUP_JB + j] = Xo[(size_t)(j0 + j) * s + k];
+        }
+        for (int e = tid; e < jn * SUP_TT; e += NT) {
+          const int j = e / SUP_TT, t = e - (e / SUP_TT) * SUP_TT;
+          const T *x = WORD ? a.vocab + (size_t)a.sym[off + j0 + j] * d
+                            : a.supp + (size_t)(off + j0 + j) * d;
+          Vl[j * SUP_TT + t] = t0 + t < d ? x[t0 + t] : (T)0;
+        }
+        __syncthreads();
+        for (int j = 0; j < jn; ++j) {
+          const double v0 = (double)Vl[j * SUP_TT + lane], v1 = (double)Vl[j * SUP_TT + 64 + lane];
 #pragma unroll
-        for (int i = 0; i < SUP_KPT; ++i) {
-          const int k = wv + 4 * i;
-          if (k < s) acc[i] += (double)xc[k] * xv;
+          for (int i = 0; i < SUP_KPT; ++i) {
+            const int k = wv + 4 * i;
+            if (k < s) {
+              const double xk = (double)Xl[k * SUP_JB + j];
+              acc[i][0] += xk * v0;
+              acc[i][1] += xk * v1;
+            }
+          }
         }
       }
     }
-    if (cur >= 0) {
-#pragma unroll
-      for (int i = 0; i < SUP_KPT; ++i) {
-        const int k = wv + 4 * i;
-        if (k < s && t < d) a.rec[(size_t)(g + cur) * R + (size_t)k * d + t] = acc[i];
-      }
-    }
+    if (cur >= 0) flush(cur);
   }
 }
 
