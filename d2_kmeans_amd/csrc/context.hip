@@ -223,6 +223,8 @@ extern "C" int d2mi_set_phase(d2mi_ctx *ctx, int phase, int metric, int dim, siz
   p.col = col;
   p.V = vocab_size;
   p.m_max = m_max;
+  p.h_pstr.assign(p_str, p_str + size);
+  p.bucket_K = 0;
   std::vector<long long> cum64(size);
   for (size_t i = 0; i < size; ++i) cum64[i] = (long long)p_str_cum[i];
   D2MI_TRY(p.p_str.assign(p_str, size * sizeof(int), s));

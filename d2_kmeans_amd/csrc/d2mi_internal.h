@@ -61,6 +61,13 @@ struct PhaseDev {
   // stored ground costs and the X of support-update passes (plan layout),
   // for phases whose cost is not recomputed inside the plan kernel
   DevMem Ctab, Xout, Z0;
+  // size buckets for batched LPs: objects grouped by support count, so each
+  // launch sizes LDS for its own largest object; jobs o*K + c per bucket
+  std::vector<int> h_pstr;
+  int bucket_K = 0;
+  std::vector<int> bucket_mmax;
+  std::vector<size_t> bucket_off;  // into bucket_map, in jobs
+  DevMem bucket_map;
 };
 
 struct d2mi_ctx {

@@ -226,6 +226,36 @@ int common_K(d2mi_ctx *ctx, int selected_phase, int *K) {
 // Sum over the selected phases of the LP values of `njobs` pairs into out
 // (zeroed here), then sqrt (clustering.c:39-127).  Job -> pair: explicit
 // arrays, or implicit (job / K, job % K) when implicit_K > 0.
+// Jobs o*K + c grouped by the support count of object o (power-of-two
+// buckets), built once per (phase, K) and kept on the device.
+int ensure_buckets(d2mi_ctx *ctx, PhaseDev &P, int K) {
+  if (P.bucket_K == K) return D2MI_OK;
+  const int N = (int)P.h_pstr.size();
+  std::vector<std::vector<int>> objs;
+  std::vector<int> mmax;
+  for (int o = 0; o < N; ++o) {
+    int b = 0;
+    while ((32 << b) < P.h_pstr[o] && b < 10) ++b;
+    if ((int)objs.size() <= b) { objs.resize(b + 1); mmax.resize(b + 1, 0); }
+    objs[b].push_back(o);
+    mmax[b] = std::max(mmax[b], P.h_pstr[o]);
+  }
+  std::vector<int> map;
+  map.reserve((size_t)N * K);
+  P.bucket_off.assign(1, 0);
+  P.bucket_mmax.clear();
+  for (size_t b = 0; b < objs.size(); ++b) {
+    if (objs[b].empty()) continue;
+    for (int o : objs[b])
+      for (int c = 0; c < K; ++c) map.push_back(o * K + c);
+    P.bucket_off.push_back(map.size());
+    P.bucket_mmax.push_back(mmax[b]);
+  }
+  D2MI_TRY(P.bucket_map.assign(map.data(), sizeof(int) * map.size(), ctx->stream));
+  P.bucket_K = K;
+  return D2MI_OK;
+}
+
 int lp_over_phases(d2mi_ctx *ctx, int selected_phase, Pairing pairing, int njobs, int implicit_K,
                    const unsigned char *mask, const int *job_a, const int *job_b, double *out) {
   D2MI_TRY(d2mi_zero_doubles(ctx, out, njobs));
@@ -275,6 +305,18 @@ int lp_over_phases(d2mi_ctx *ctx, int selected_phase, Pairing pairing, int njobs
       a.sym = (P.metric == D2MI_SPARSE_HISTOGRAM || P.metric == D2MI_WORD_EMBED)
                   ? P.sym.as<int>() : nullptr;
       a.m_max = P.m_max;
+      if (implicit_K > 0 && njobs == (int)P.N * implicit_K && P.metric != D2MI_HISTOGRAM) {
+        // one launch per size bucket, LDS and workgroup size fitted to it
+        D2MI_TRY(ensure_buckets(ctx, P, implicit_K));
+        for (size_t b = 0; b + 1 < P.bucket_off.size(); ++b) {
+          LpArgs ab = a;
+          ab.njobs = (int)(P.bucket_off[b + 1] - P.bucket_off[b]);
+          ab.job_map = P.bucket_map.as<int>() + P.bucket_off[b];
+          ab.m_max = P.bucket_mmax[b];
+          D2MI_TRY(d2mi_launch_transport(ctx, ab));
+        }
+        continue;
+      }
     } else {
       a.b_is_table = 1;
       a.sB = P.s;

@@ -20,6 +20,8 @@ struct LpArgs {
   const int *job_a;
   const int *job_b;
   int implicit_K;              // > 0: job -> (job_b = job / K, job_a = job % K), no job arrays
+  const int *job_map;          // with implicit_K: launch job t is global job job_map[t]
+                               // (mask, out and the pair are those of the global job)
   const unsigned char *mask;   // optional: jobs with mask[job] == 0 are skipped
   int b_is_table;
   int cost_mode;

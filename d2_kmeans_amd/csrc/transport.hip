@@ -25,7 +25,8 @@ __global__ __launch_bounds__(1024) void k_transport(LpArgs a) {
   double *A = (double *)(lds + L.A), *B = (double *)(lds + L.B);
   double *wa = (double *)(lds + L.wa), *wb = (double *)(lds + L.wb);
   int *ids = (int *)(lds + L.ids);
-  for (int job = blockIdx.x; job < a.njobs; job += gridDim.x) {
+  for (int t = blockIdx.x; t < a.njobs; t += gridDim.x) {
+    const int job = a.job_map ? a.job_map[t] : t;
     if (a.mask && a.mask[job] == 0) continue;
     // row side: centroid `ka` of table A
     const int ka = a.implicit_K > 0 ? job % a.implicit_K : a.job_a[job];
@@ -144,10 +145,14 @@ int d2mi_plan_transport(d2mi_ctx *ctx, LpArgs &a) {
   a.lds = bytes(tab, sa, sb);
   // many small LPs: one wave each; few or large LPs: more waves per LP
   const size_t cells = (size_t)a.n_max * a.m_max;
+  // few LPs, or LDS for only one or two per CU: more waves per LP
+  const size_t per_cu = std::max<size_t>(1, cap / a.lds);
   int T = 64;
-  if (a.njobs < ctx->num_cu * 4) {
+  if (a.njobs < ctx->num_cu * 4 || per_cu <= 2) {
     if (cells >= 8192) T = 256;
     else if (cells >= 2048) T = 128;
+  } else if (per_cu <= 4 && cells >= 2048) {
+    T = 128;
   }
   if (a.threads_hint > 0) T = a.threads_hint;
   T = env_int("D2MI_LP_THREADS", T);
