@@ -244,37 +244,10 @@ def word_embed_leg(eng):
     return out
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--n", type=int, default=200_000)
-    ap.add_argument("--cpu-sample", type=int, default=6000)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--no-gate", action="store_true")
-    ap.add_argument("--no-assign", action="store_true")
-    ap.add_argument("--no-word-embed", action="store_true")
-    ap.add_argument("--assign-n", type=int, default=3000)
-    ap.add_argument("--assign-qps", type=float, default=1000.0)
-    ap.add_argument("--assign-plain-n", type=int, default=300)
-    ap.add_argument("--assign-check", type=int, default=100)
-    ap.add_argument("--assign-train", type=int, default=4000)
-    ap.add_argument("--assign-train-iters", type=int, default=6)
-    ap.add_argument("--copy-calib", action="store_true",
-                    help="also run one FP32->FP64 plan copy kernel (reads 4 B, writes 8 B per "
-                         "plan element) to scale rocprof byte counters")
-    args = ap.parse_args()
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(see the sharding design); run with --gpus 1", file=sys.stderr)
-        sys.exit(2)
-
+def headline_leg(eng, args):
+    """B-ADMM plan updates/s on the image-like workload, gated against FP64."""
     from d2_kmeans_amd import synth
     from d2_kmeans_amd.data import BADMMOptions
-    from d2_kmeans_amd.engine import Engine
-
     K, s = 10, 40
     opts = BADMMOptions(10, 2.0, 10)
     log(f"[bench] generating grid-strokes workload N={args.n}")
@@ -282,7 +255,6 @@ def main():
     labels = meta["classes"]
     ph = data.phases[0]
     c0 = synth.seed_centroids(ph, K, s, labels=labels)
-    eng = Engine(0)
     eng.set_data(data)
 
     def reset(precision, lsw_all=True):
@@ -358,27 +330,69 @@ def main():
         cpu = cpu_baseline(data, labels, c0, opts, args.cpu_sample)
         log(f"[bench] cpu baseline: {cpu}")
 
+    return dict(value=value, ms_per_step=ms_per_step, ph=ph, K=K, s=s, elems=elems,
+                achieved=achieved, avg_plan_ms=avg_plan_ms, bytes_per_launch=bytes_per_launch,
+                gate=gate, plain=plain, cpu=cpu)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=200_000)
+    ap.add_argument("--cpu-sample", type=int, default=6000)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-gate", action="store_true")
+    ap.add_argument("--skip-badmm", action="store_true",
+                    help="skip the headline leg (for profiling the other legs alone)")
+    ap.add_argument("--no-assign", action="store_true")
+    ap.add_argument("--no-word-embed", action="store_true")
+    ap.add_argument("--assign-n", type=int, default=3000)
+    ap.add_argument("--assign-qps", type=float, default=1000.0)
+    ap.add_argument("--assign-plain-n", type=int, default=300)
+    ap.add_argument("--assign-check", type=int, default=100)
+    ap.add_argument("--assign-train", type=int, default=4000)
+    ap.add_argument("--assign-train-iters", type=int, default=6)
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="also run one FP32->FP64 plan copy kernel (reads 4 B, writes 8 B per "
+                         "plan element) to scale rocprof byte counters")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(see the sharding design); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from d2_kmeans_amd.engine import Engine
+
+    eng = Engine(0)
+    h = None if args.skip_badmm else headline_leg(eng, args)
     assign = None if args.no_assign else assign_leg(eng, args)
     word = None if args.no_word_embed else word_embed_leg(eng)
 
+    hd = h or {}
+    ph = hd.get("ph")
     out = {
-        "metric": METRIC, "value": value, "unit": "plan updates/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
+        "metric": METRIC, "value": hd.get("value"), "unit": "plan updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": hd.get("ms_per_step"),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
         "data": "synthetic",
         "config": {"workload": "grid-strokes D2, 1 Euclidean phase d=2, 16x16 grid, "
-                               f"N={ph.size}, m in [20,80], K={K}, s={s}, B-ADMM maxIters=10 "
+                               f"N={args.n}, m in [20,80], K=10, s=40, B-ADMM maxIters=10 "
                                "(11 passes), rhoCoeff=2.0, supports every 10 passes",
-                   "objects": ph.size, "plan_elements": int(elems), "K": K, "s": s},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBPS, "traffic": profiled_traffic(),
-                     "kernel": "k_badmm_plan<float,EUCLID>",
-                     "kernel_ms_avg": avg_plan_ms, "algorithmic_bytes_per_launch": bytes_per_launch},
-        "gate_obj_rel_err": None if gate is None else gate["obj_rel_err"],
-        "gate": gate,
-        "plain_form": plain,
-        "speedup_vs_plain": None if plain is None else value / plain["value"],
-        "cpu_baseline": cpu,
+                   "objects": args.n, "plan_elements": int(hd["elems"]) if h else None,
+                   "K": 10, "s": 40},
+        "roofline": None if h is None else {
+            "bound": "hbm", "achieved": hd["achieved"], "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+            "frac": hd["achieved"] / HBM_PEAK_GBPS, "traffic": profiled_traffic(),
+            "kernel": "k_badmm_plan<float,EUCLID>", "kernel_ms_avg": hd["avg_plan_ms"],
+            "algorithmic_bytes_per_launch": hd["bytes_per_launch"]},
+        "gate_obj_rel_err": None if not hd.get("gate") else hd["gate"]["obj_rel_err"],
+        "gate": hd.get("gate"),
+        "plain_form": hd.get("plain"),
+        "speedup_vs_plain": None if not hd.get("plain") else hd["value"] / hd["plain"]["value"],
+        "cpu_baseline": hd.get("cpu"),
         "assign": assign,
         "word_embed": word,
     }

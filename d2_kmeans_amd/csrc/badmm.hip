@@ -161,15 +161,21 @@ __device__ __forceinline__ T scale_cost(T c, double rho, T inv_rho) {
 constexpr int PRE_W = 2;  // register slots for weights / bin ids: m_max <= 512
 constexpr int PRE_X = 4;  // register slots for supports: m_max * d <= 1024
 
+#ifndef D2MI_PLAN_UNROLL
+#define D2MI_PLAN_UNROLL 4
+#endif
+constexpr int PLAN_UNROLL = D2MI_PLAN_UNROLL;  // plan elements per thread per load batch
+
 #ifndef D2MI_PLAN_WAVES_PER_EU
 #define D2MI_PLAN_WAVES_PER_EU 1
 #endif
 
 // Plan update of every object in this workgroup's chunk of the label-sorted
-// order.  Per object: its Z, Y (RMAX register slots of NT elements; RMAX == 0
-// loads inside the first pass), weights and supports are requested at the
-// top, before the workgroup barrier, so their HBM round trips overlap; then
-//   pass 1 (element per thread): t = Z exp(-(C + Y)) + 1e-9 into LDS;
+// order (any s; k_badmm_plan64 is the faster form for s <= 64 with the plan
+// in LDS).  Per object: weights and supports are requested at the top, before
+// the workgroup barrier, so their HBM round trips overlap; then
+//   pass 1 (element per thread, PLAN_UNROLL elements' loads in flight at
+//          once): t = Z exp(-(C + Y)) + 1e-9 into LDS;
 //   pass 2 (four threads per column): column sums by quad shuffles, then the
 //          same threads form x = t / colsum * w and z' = x exp(Y) + 1e-9;
 //   pass 3 (four waves): partial row sums, combined by wave 0 into the
@@ -180,17 +186,22 @@ constexpr int PRE_X = 4;  // register slots for supports: m_max * d <= 1024
 // scratch in HBM -- t and z' in place in Z, y (+ x) in place in Y, x in a
 // scratch array; the workgroup barriers order these accesses as they order
 // the LDS ones.
-template <typename T, int CM, int RMAX, int DIM, bool GM = false>
+//
+// LEAN instances serve the passes after the first with no support update,
+// residual or kept X (most passes): those branches are compiled out, which
+// frees registers for more resident waves.
+template <typename T, int CM, int DIM, bool GM = false, bool LEAN = false>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(D2MI_PLAN_WAVES_PER_EU)))
 void k_badmm_plan(PlanArgs<T> a, const int *__restrict__ g_perm, const int *__restrict__ g_str,
                   const long long *__restrict__ g_cum, const int *__restrict__ g_label,
                   const char *__restrict__ g_lsw) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr bool F64 = std::is_same<T, double>::value;
-  constexpr int RS = RMAX > 0 ? RMAX : 1;
   const int s = a.s, d = DIM > 0 ? DIM : a.d, sp = GM ? a.s : a.sp, tid = threadIdx.x;
   const int lane = tid & 63, wv = tid >> 6;
-  const PlanLds L(sizeof(T), a.sp, a.m_max, s, d, a.keepx, a.R, GM);
+  const bool upd = !LEAN && a.upd, resid = !LEAN && a.resid, keepx = !LEAN && a.keepx;
+  const bool first = !LEAN && a.first;
+  const PlanLds L(sizeof(T), a.sp, a.m_max, s, d, keepx, a.R, GM);
   T *A = (T *)(smem + L.A), *B = (T *)(smem + L.B), *Xs = (T *)(smem + L.X);
   T *wl = (T *)(smem + L.w), *xs = (T *)(smem + L.xs);
   T *cs = (T *)(smem + L.cs), *cwl = (T *)(smem + L.cw), *g = (T *)(smem + L.g);
@@ -223,20 +234,10 @@ void k_badmm_plan(PlanArgs<T> a, const int *__restrict__ g_perm, const int *__re
     }
     // first pass of an update: Y = 0 (:128) and, for objects whose label
     // switched, Z = w_j * c_w[label][k] (:116-124), without touching HBM
-    const bool reinit = a.first && (a.force_init || g_lsw[o] == 1);
-    // requests first: plan, weights, supports of this object
-    T zr[RS], yr[RS], wpre[PRE_W], xpre[PRE_X];
+    const bool reinit = first && (a.force_init || g_lsw[o] == 1);
+    // requests first: weights and supports of this object
+    T wpre[PRE_W], xpre[PRE_X];
     int ipre[PRE_W];
-    if constexpr (RMAX > 0) {
-#pragma unroll
-      for (int r = 0; r < RMAX; ++r) {
-        const int e = tid + r * NT;
-        if (e < E) {
-          if (!reinit) zr[r] = a.Z[base + e];
-          yr[r] = a.first ? (T)0 : a.Y[base + e];
-        }
-      }
-    }
 #pragma unroll
     for (int r = 0; r < PRE_W; ++r) {
       const int q = tid + r * NT;
@@ -284,37 +285,46 @@ void k_badmm_plan(PlanArgs<T> a, const int *__restrict__ g_perm, const int *__re
     // pass 1 -- step 1a: t = Z exp(-(C + Y)) + 1e-9 (:155-157)
     {
       int k = k0, j = j0;
-      if constexpr (RMAX > 0) {
+      {
+        // U elements per thread at a time: all their loads are in flight
+        // before the first is used, instead of one HBM round trip each
+        constexpr int U = PLAN_UNROLL;
+        for (int e0 = tid; e0 < E; e0 += U * NT) {
+          T zv[U], yv[U], cv[U];
+          {
+            int kk = k, jj = j;
 #pragma unroll
-        for (int r = 0; r < RMAX; ++r) {
-          const int e = tid + r * NT;
-          if (e < E) {
-            const T c = scale_cost<T>(
-                cost_raw<T, CM, DIM>(k, j, s, d, a.V, cs, xs, ids, a.dist, a.ctab + base), rho,
-                inv_rho);
-            const int li = k + j * sp;
-            const T z = reinit ? wl[j] * cwl[k] : zr[r];
-            A[li] = z * dexp<T>(-(c + yr[r])) + (T)1e-9;
-            B[li] = yr[r];
+            for (int u = 0; u < U; ++u) {
+              const int e = e0 + u * NT;
+              zv[u] = (T)0;
+              yv[u] = (T)0;
+              cv[u] = (T)0;
+              if (e < E) {
+                if (!reinit) zv[u] = a.Z[base + e];
+                if (!first) yv[u] = a.Y[base + e];
+                cv[u] = cost_raw<T, CM, DIM>(kk, jj, s, d, a.V, cs, xs, ids, a.dist,
+                                             a.ctab + base);
+              }
+              kk += dk;
+              jj += dj;
+              if (kk >= s) { kk -= s; ++jj; }
+            }
           }
-          k += dk;
-          j += dj;
-          if (k >= s) { k -= s; ++j; }
-        }
-      } else {
-        for (int e = tid; e < E; e += NT) {
-          const T c = scale_cost<T>(
-              cost_raw<T, CM, DIM>(k, j, s, d, a.V, cs, xs, ids, a.dist, a.ctab + base), rho,
-              inv_rho);
-          const T z = reinit ? wl[j] * cwl[k] : a.Z[base + e];
-          const T y = a.first ? (T)0 : a.Y[base + e];
-          const int li = k + j * sp;
-          if (GM && a.resid) a.z0g[base + e] = z;
-          A[li] = z * dexp<T>(-(c + y)) + (T)1e-9;
-          B[li] = y;
-          k += dk;
-          j += dj;
-          if (k >= s) { k -= s; ++j; }
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            const int e = e0 + u * NT;
+            if (e < E) {
+              const T c = scale_cost<T>(cv[u], rho, inv_rho);
+              const T z = reinit ? wl[j] * cwl[k] : zv[u];
+              const int li = k + j * sp;
+              if (GM && resid) a.z0g[base + e] = z;
+              A[li] = z * dexp<T>(-(c + yv[u])) + (T)1e-9;
+              B[li] = yv[u];
+            }
+            k += dk;
+            j += dj;
+            if (k >= s) { k -= s; ++j; }
+          }
         }
       }
     }
@@ -335,7 +345,7 @@ void k_badmm_plan(PlanArgs<T> a, const int *__restrict__ g_perm, const int *__re
         const T x = F64 ? (colA[k] / part) * wj : colA[k] * scale;
         const T y = B[li];
         colA[k] = x * dexp<T>(y) + (T)1e-9;
-        if (a.keepx) Xs[li] = x;
+        if (keepx) Xs[li] = x;
         else B[li] = y + x;
       }
     }
@@ -355,7 +365,7 @@ void k_badmm_plan(PlanArgs<T> a, const int *__restrict__ g_perm, const int *__re
         rr[k] = r;
       }
     }
-    if (a.upd) {
+    if (upd) {
       if (small_s) {  // wave wv: columns j = wv, wv + 4, ...
         if (lane < s) {
           double P[MAXD_FUSED];
@@ -397,7 +407,7 @@ void k_badmm_plan(PlanArgs<T> a, const int *__restrict__ g_perm, const int *__re
         }
         double rs = wave_sum(lane < s ? (double)r : 0.0);
         if (lane < s) acc[lane] += (double)r / rs;
-        if (a.upd && lane < s) {
+        if (upd && lane < s) {
           for (int dd = 0; dd <= d; ++dd) {
             double v = 0.0;
             for (int w4 = 0; w4 < 4; ++w4) v += up[(size_t)(w4 * s + lane) * (d + 1) + dd];
@@ -422,7 +432,7 @@ void k_badmm_plan(PlanArgs<T> a, const int *__restrict__ g_perm, const int *__re
       for (int e = tid; e < E; e += NT) {
         const int li = k + j * sp;
         const T z = F64 ? (A[li] / g[k]) * cwl[k] : A[li] * g[k];
-        if (a.resid) {  // step 6 (:300-305), and the exit objective
+        if (resid) {  // step 6 (:300-305), and the exit objective
           const T c = scale_cost<T>(
               cost_raw<T, CM, DIM>(k, j, s, d, a.V, cs, xs, ids, a.dist, a.ctab + base), rho,
               inv_rho);
@@ -433,19 +443,706 @@ void k_badmm_plan(PlanArgs<T> a, const int *__restrict__ g_perm, const int *__re
           r_dr += fabs((double)z0 - (double)z);
         }
         a.Z[base + e] = z;
-        a.Y[base + e] = a.keepx ? B[li] + (Xs[li] - z) : B[li] - z;
-        if (CM == COST_TABLE && a.upd && !GM) a.xout[base + e] = Xs[li];  // for the support GEMM
+        a.Y[base + e] = keepx ? B[li] + (Xs[li] - z) : B[li] - z;
+        if (CM == COST_TABLE && upd && !GM) a.xout[base + e] = Xs[li];  // for the support GEMM
         k += dk;
         j += dj;
         if (k >= s) { k -= s; ++j; }
       }
     }
-    if (a.resid) __syncthreads();  // pass 4 read xs / cs: keep them until all waves are done
+    if (resid) __syncthreads();  // pass 4 read xs / cs: keep them until all waves are done
   }
   __syncthreads();
   if (cur >= 0)
     for (int q = tid; q < a.R; q += NT) a.rec[(size_t)(gi + cur) * a.R + q] = acc[q];
-  if (a.resid) {
+  if (resid) {
+    const double t0 = block_sum(r_obj, red);
+    const double t1 = block_sum(r_pr, red);
+    const double t2 = block_sum(r_dr, red);
+    if (tid == 0) {
+      a.resid_part[gi * 3 + 0] = t0;
+      a.resid_part[gi * 3 + 1] = t1;
+      a.resid_part[gi * 3 + 2] = t2;
+    }
+  }
+}
+
+// ------------------------------------------------- plan kernel for s <= 64
+
+// FP32 exp(x) on the hardware base-2 exponential: exp(x) = 2^p 2^r with
+// p = fl(x log2 e) and r = x log2 e - p recovered exactly enough by two FMAs
+// (the high part of log2 e is its float rounding, the low part the rest), then
+// 2^r = 1 + r ln 2 to first order (|r| is below half an ulp of p).  Relative
+// error within a few ulps over the plan's argument range.
+__device__ __forceinline__ float exp_split(float x) {
+  constexpr float L2E_HI = 1.44269502162933349609375f;  // log2(e) rounded to float
+  constexpr float L2E_LO = 1.925963033500011e-8f;       // log2(e) - L2E_HI
+  constexpr float LN2 = 0.693147180559945309f;
+  const float p = x * L2E_HI;
+  const float r = fmaf(x, L2E_LO, fmaf(x, L2E_HI, -p));
+  return __builtin_amdgcn_exp2f(p) * fmaf(r, LN2, 1.0f);
+}
+
+template <typename T>
+__device__ __forceinline__ T plan_exp(T x) {
+  if constexpr (std::is_same<T, double>::value) return exp(x);
+  else return exp_split(x);
+}
+
+// LDS layout of k_badmm_plan64, shared by host (sizing) and device.  Plan
+// columns have stride sp (even, >= s): rows 2i, 2i + 1 of a column are one
+// 8-byte (FP32) or 16-byte (FP64) word.
+struct Plan64Lds {
+  size_t A, B, X, w, xs, cs, cw, g, rp, ids, acc, red, up, total;
+  __host__ __device__ Plan64Lds(int tsize, int sp, int m_max, int s, int d, int keepx, int R) {
+    const size_t EP = (size_t)sp * m_max, dd = d > 0 ? d : 1;
+    size_t o = 0;
+    A = o; o = align16(o + EP * tsize);
+    B = o; o = align16(o + EP * tsize);
+    X = o; o = align16(o + (keepx ? EP : 0) * tsize);
+    w = o; o = align16(o + (size_t)m_max * tsize);
+    xs = o; o = align16(o + (size_t)m_max * dd * tsize);
+    cs = o; o = align16(o + (size_t)s * dd * tsize);
+    cw = o; o = align16(o + (size_t)s * tsize);
+    g = o; o = align16(o + (size_t)s * tsize);
+    rp = o; o = align16(o + (size_t)NT * tsize);
+    ids = o; o = align16(o + (size_t)m_max * sizeof(int));
+    acc = o; o = align16(o + (size_t)R * sizeof(double));
+    red = o; o = align16(o + 8 * sizeof(double));
+    up = o; o = align16(o + (R > s ? (size_t)4 * s * (dd + 1) : 0) * sizeof(double));
+    total = o;
+  }
+};
+
+template <typename T>
+struct Vec2 {
+  typedef float2 type;
+};
+template <>
+struct Vec2<double> {
+  typedef double2 type;
+};
+
+// The plan update of k_badmm_plan, restated for s <= 64 with fewer
+// instructions per plan element:
+//   * element passes (1 and 4) give each thread a fixed support row
+//     k = tid % s and columns j = tid / s + c * (NT / s), so the element
+//     offset k + j s runs contiguously over the workgroup (coalesced), the
+//     centroid support, c_w[k] and the row normaliser sit in registers and no
+//     (k, j) stepping is needed;
+//   * pass 1 loads U columns' Z, Y (and stored costs) before using any;
+//   * pass 2 gives each of four threads of a column a contiguous run of RQ
+//     rows, read and written as 2-row LDS words and kept in registers between
+//     the column sum and the update;
+//   * pass 3 sums rows over all NT / s column classes in parallel.
+// Every sum keeps a fixed order, so results are reproducible run to run.
+// register budget: the lean passes are the bulk of the time and should keep
+// five or more waves per SIMD (about the LDS limit for a typical object); the
+// support-update and residual passes may use more registers
+#ifndef D2MI_PLAN64_WAVES_PER_EU
+#define D2MI_PLAN64_WAVES_PER_EU 5
+#endif
+template <typename T, int CM, int DIM, bool LEAN>
+__global__ __launch_bounds__(NT)
+__attribute__((amdgpu_waves_per_eu(LEAN ? D2MI_PLAN64_WAVES_PER_EU : 1)))
+void k_badmm_plan64(PlanArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr bool F64 = std::is_same<T, double>::value;
+  constexpr int U = PLAN_UNROLL;
+  typedef typename Vec2<T>::type T2;
+  const int s = a.s, d = DIM > 0 ? DIM : a.d, sp = a.sp, tid = threadIdx.x;
+  const int lane = tid & 63, wv = tid >> 6;
+  const bool upd = !LEAN && a.upd, resid = !LEAN && a.resid, keepx = !LEAN && a.keepx;
+  const bool first = !LEAN && a.first;
+  const Plan64Lds L(sizeof(T), sp, a.m_max, s, d, keepx, a.R);
+  T *A = (T *)(smem + L.A), *B = (T *)(smem + L.B), *Xs = (T *)(smem + L.X);
+  T *wl = (T *)(smem + L.w), *xs = (T *)(smem + L.xs);
+  T *cs = (T *)(smem + L.cs), *cwl = (T *)(smem + L.cw), *g = (T *)(smem + L.g);
+  T *rp = (T *)(smem + L.rp);
+  int *ids = (int *)(smem + L.ids);
+  double *acc = (double *)(smem + L.acc), *red = (double *)(smem + L.red);
+  double *up = (double *)(smem + L.up);
+
+  const int gi = blockIdx.x;
+  const int p0 = gi * a.chunk, p1 = min(a.N, p0 + a.chunk);
+  const double rho = a.scal[0];
+  const T inv_rho = (T)a.scal[1];
+  // element passes: fixed row ke, columns jb, jb + cpi, ...
+  const int cpi = NT / s, ke = tid % s;
+  const int jb = tid < cpi * s ? tid / s : 0x3fffffff;  // idle threads see no column
+  // pass 2: column (tid >> 2) + 64 c, rows [q RQ, q RQ + RQ) of it
+  const int q4 = tid & 3, RQ = (((s + 3) >> 2) + 1) & ~1, r0 = q4 * RQ;
+  double r_obj = 0.0, r_pr = 0.0, r_dr = 0.0;
+  for (int q = tid; q < a.R; q += NT) acc[q] = 0.0;
+  int cur = -1;
+
+  for (int p = p0; p < p1; ++p) {
+    const int o = a.perm[p];
+    const int l = a.label[o];
+    const int m = a.p_str[o];
+    const long long off = a.cum[o];
+    const size_t base = (size_t)s * off;
+    const bool reinit = first && (a.force_init || a.lsw[o] == 1);
+    // requests first: weights, supports or bin ids of this object
+    T wpre[PRE_W], xpre[PRE_X];
+    int ipre[PRE_W];
+#pragma unroll
+    for (int r = 0; r < PRE_W; ++r) {
+      const int q = tid + r * NT;
+      if (q < m) {
+        wpre[r] = a.w[off + q];
+        if constexpr (CM == COST_SPARSE) ipre[r] = a.sym[off + q];
+      }
+    }
+    if constexpr (CM == COST_EUCLID) {
+#pragma unroll
+      for (int r = 0; r < PRE_X; ++r) {
+        const int q = tid + r * NT;
+        if (q < m * d) xpre[r] = a.supp[(size_t)off * d + q];
+      }
+    }
+    if (l != cur) {
+      __syncthreads();  // every wave is done with the previous label
+      if (cur >= 0)
+        for (int q = tid; q < a.R; q += NT) {
+          a.rec[(size_t)(gi + cur) * a.R + q] = acc[q];
+          acc[q] = 0.0;
+        }
+      cur = l;
+      for (int q = tid; q < s; q += NT) cwl[q] = a.cw[(size_t)l * s + q];
+      if constexpr (CM == COST_EUCLID)
+        for (int q = tid; q < s * d; q += NT) cs[q] = a.csupp[(size_t)l * s * d + q];
+    }
+#pragma unroll
+    for (int r = 0; r < PRE_W; ++r) {
+      const int q = tid + r * NT;
+      if (q < m) {
+        wl[q] = wpre[r];
+        if constexpr (CM == COST_SPARSE) ids[q] = ipre[r];
+      }
+    }
+    if constexpr (CM == COST_EUCLID) {
+#pragma unroll
+      for (int r = 0; r < PRE_X; ++r) {
+        const int q = tid + r * NT;
+        if (q < m * d) xs[q] = xpre[r];
+      }
+    }
+    __syncthreads();  // also: every wave has finished the previous object's last pass
+    const T cwk = cwl[ke];
+    T csk[DIM > 0 ? DIM : 1];
+    if constexpr (CM == COST_EUCLID && DIM > 0) {
+#pragma unroll
+      for (int dd = 0; dd < DIM; ++dd) csk[dd] = cs[ke * DIM + dd];
+    }
+    T *Zo = a.Z + base, *Yo = a.Y + base;  // this object's plan (offsets k + j s)
+    // ground cost of (ke, j) before the division by rho; stored and table
+    // costs come from the loads issued with the batch
+    auto cost_k = [&](int j) -> T {
+      if constexpr (CM == COST_EUCLID) {
+        T c = 0;
+        if constexpr (DIM > 0) {
+#pragma unroll
+          for (int dd = 0; dd < DIM; ++dd) {
+            const T u = csk[dd] - xs[j * DIM + dd];
+            c += u * u;
+          }
+        } else {
+          for (int dd = 0; dd < d; ++dd) {
+            const T u = cs[ke * d + dd] - xs[j * d + dd];
+            c += u * u;
+          }
+        }
+        return c;
+      } else if constexpr (CM == COST_HIST) {
+        return a.dist[ke + (size_t)j * s];
+      } else if constexpr (CM == COST_SPARSE) {
+        return a.dist[(size_t)ids[j] * a.V + ke];
+      } else {
+        return a.ctab[base + ke + j * s];
+      }
+    };
+
+    // pass 1 -- step 1a: t = Z exp(-(C + Y)) + 1e-9 (:155-157)
+    for (int j0 = jb; j0 < m; j0 += U * cpi) {
+      T zv[U], yv[U], cv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int jc = min(j0 + u * cpi, m - 1);  // in bounds; results past m unused
+        const int e = ke + jc * s;
+        zv[u] = reinit ? (T)0 : Zo[e];
+        yv[u] = first ? (T)0 : Yo[e];
+        cv[u] = cost_k(jc);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int j = j0 + u * cpi;
+        if (j < m) {
+          const T z = reinit ? wl[j] * cwk : zv[u];
+          const int li = ke + j * sp;
+          A[li] = z * plan_exp<T>(-(scale_cost<T>(cv[u], rho, inv_rho) + yv[u])) + (T)1e-9;
+          B[li] = yv[u];
+        }
+      }
+    }
+    __syncthreads();
+    // pass 2 -- step 1b/1c: column sums (_dcnorm, :158), x = t / colsum * w
+    // (:159); step 2a: z' = x exp(Y) + 1e-9 (:166)
+    {
+      const int nr = max(0, min(RQ, s - r0));  // rows of this thread in its column
+      for (int j = tid >> 2; j < m; j += NT / 4) {
+        T *colA = A + j * sp + r0;
+        T part = 0;
+        for (int r = 0; r < nr; r += 2) {
+          const T2 v = *(const T2 *)(colA + r);
+          part += v.x;
+          if (r + 1 < nr) part += v.y;
+        }
+        part += __shfl_xor(part, 1, 64);
+        part += __shfl_xor(part, 2, 64);
+        const T wj = wl[j];
+        const T scale = F64 ? (T)0 : wj / part;
+        T *colB = B + j * sp + r0;
+        for (int r = 0; r < nr; r += 2) {
+          const T2 t = *(const T2 *)(colA + r);
+          const T2 y = *(const T2 *)(colB + r);
+          T2 x, zp;
+          x.x = F64 ? (t.x / part) * wj : t.x * scale;
+          x.y = F64 ? (t.y / part) * wj : t.y * scale;
+          zp.x = x.x * plan_exp<T>(y.x) + (T)1e-9;
+          zp.y = x.y * plan_exp<T>(y.y) + (T)1e-9;
+          *(T2 *)(colA + r) = zp;  // a pad row past an odd s is written, never read
+          if (keepx) {
+            *(T2 *)(Xs + j * sp + r0 + r) = x;
+          } else {
+            T2 yx;
+            yx.x = y.x + x.x;
+            yx.y = y.y + x.y;
+            *(T2 *)(colB + r) = yx;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    // pass 3 -- step 2b: row sums (_drnorm, :169), by column class; step 5
+    // partials on support-update iterations (:206-211), by wave
+    {
+      T part = 0;
+      for (int j = jb; j < m; j += cpi) part += A[ke + j * sp];
+      if (jb < cpi) rp[jb * s + ke] = part;
+    }
+    if (upd && lane < s) {  // wave wv: columns j = wv, wv + 4, ...
+      double P[MAXD_FUSED];
+      for (int dd = 0; dd < d; ++dd) P[dd] = 0.0;
+      double rx = 0.0;
+      for (int j = wv; j < m; j += 4) {
+        const double x = (double)Xs[lane + j * sp];
+        rx += x;  // row sums of X (:211)
+        if constexpr (CM == COST_EUCLID)
+          for (int dd = 0; dd < d; ++dd) P[dd] += x * (double)xs[j * d + dd];  // supp X^T (:206)
+      }
+      double *u = up + (size_t)(wv * s + lane) * (d + 1);
+      for (int dd = 0; dd < d; ++dd) u[dd] = P[dd];
+      u[d] = rx;
+    }
+    __syncthreads();
+    // step 4: normalised row sums summed per label (:180-185)
+    if (wv == 0) {
+      T r = 0;
+      if (lane < s) {
+        for (int c = 0; c < cpi; ++c) r += rp[c * s + lane];
+        g[lane] = F64 ? r : cwl[lane] / r;
+      }
+      const double rs = wave_sum(lane < s ? (double)r : 0.0);
+      if (lane < s) acc[lane] += (double)r / rs;
+      if (upd && lane < s) {
+        for (int dd = 0; dd <= d; ++dd) {
+          double v = 0.0;
+          for (int w4 = 0; w4 < 4; ++w4) v += up[(size_t)(w4 * s + lane) * (d + 1) + dd];
+          if (dd < d) acc[s + lane * d + dd] += v;
+          else acc[s + s * d + lane] += v;
+        }
+      }
+    }
+    __syncthreads();
+    // pass 4 -- step 2c + 3: z = z' / rowsum * c_w (:169-170); Y += X - Z (:175)
+    {
+      const T gk = g[ke];
+      for (int j0 = jb; j0 < m; j0 += U * cpi) {
+        T z0v[U], cv[U];
+        if (resid) {  // step 6 (:300-305) and the exit objective: Z before this pass
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            const int jc = min(j0 + u * cpi, m - 1);
+            z0v[u] = reinit ? wl[jc] * cwk : Zo[ke + jc * s];
+            cv[u] = cost_k(jc);
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int j = j0 + u * cpi;
+          if (j < m) {
+            const int li = ke + j * sp, e = ke + j * s;
+            const T z = F64 ? (A[li] / gk) * cwk : A[li] * gk;
+            if (resid) {
+              const T c = scale_cost<T>(cv[u], rho, inv_rho);
+              const T x = Xs[li];
+              r_obj += (double)c * (double)x;
+              r_pr += fabs((double)x - (double)z);
+              r_dr += fabs((double)z0v[u] - (double)z);
+            }
+            Zo[e] = z;
+            Yo[e] = keepx ? B[li] + (Xs[li] - z) : B[li] - z;
+            if (CM == COST_TABLE && upd) a.xout[base + e] = Xs[li];  // for the support GEMM
+          }
+        }
+      }
+    }
+    if (resid) __syncthreads();  // pass 4 read wl / xs: keep them until all waves are done
+  }
+  __syncthreads();
+  if (cur >= 0)
+    for (int q = tid; q < a.R; q += NT) a.rec[(size_t)(gi + cur) * a.R + q] = acc[q];
+  if (resid) {
+    const double t0 = block_sum(r_obj, red);
+    const double t1 = block_sum(r_pr, red);
+    const double t2 = block_sum(r_dr, red);
+    if (tid == 0) {
+      a.resid_part[gi * 3 + 0] = t0;
+      a.resid_part[gi * 3 + 1] = t1;
+      a.resid_part[gi * 3 + 2] = t2;
+    }
+  }
+}
+
+// ------------------------------------ register-resident plan kernel (s <= 64)
+
+// register budget of the lean passes: 6 waves per SIMD with 8 column slots,
+// 5 with 16 (the slots alone take 32 registers)
+#ifndef D2MI_PLANREG_WAVES_PER_EU
+#define D2MI_PLANREG_WAVES_PER_EU(nc) ((nc) <= 8 ? 6 : 5)
+#endif
+#ifndef D2MI_PLANREG_RESID_WAVES
+#define D2MI_PLANREG_RESID_WAVES 4  // residual passes: 4 waves per SIMD (a small spill)
+#endif
+#ifndef D2MI_PLANREG_MID_CAP
+#define D2MI_PLANREG_MID_CAP 1  // FP32 support-update passes share the lean budget
+#endif
+
+// A uniform read of data the kernel never writes, through the constant
+// address space, so it is a scalar load: it then waits on its own counter,
+// not behind the vector stores of the previous object.
+template <typename T>
+__device__ __forceinline__ T const_load(const T *p) {
+  return *(const __attribute__((address_space(4))) T *)p;
+}
+
+// LDS layout of k_badmm_plan_reg: only t goes through LDS (for the column
+// sums); column stride sp even, as for k_badmm_plan64.
+struct PlanRegLds {
+  size_t A, col, w, xs, cs, cw, g, rp, ids, acc, red, up, total;
+  __host__ __device__ PlanRegLds(int tsize, int sp, int m_max, int s, int d, int R) {
+    const size_t dd = d > 0 ? d : 1;
+    const int cpi = NT / s;
+    size_t o = 0;
+    A = o; o = align16(o + (size_t)sp * m_max * tsize);
+    col = o; o = align16(o + (size_t)m_max * tsize);
+    w = o; o = align16(o + (size_t)m_max * tsize);
+    xs = o; o = align16(o + (size_t)m_max * dd * tsize);
+    cs = o; o = align16(o + (size_t)s * dd * tsize);
+    cw = o; o = align16(o + (size_t)s * tsize);
+    g = o; o = align16(o + (size_t)s * tsize);
+    rp = o; o = align16(o + (size_t)NT * tsize);
+    ids = o; o = align16(o + (size_t)m_max * sizeof(int));
+    acc = o; o = align16(o + (size_t)R * sizeof(double));
+    red = o; o = align16(o + 8 * sizeof(double));
+    up = o; o = align16(o + (R > s ? (size_t)cpi * s * (dd + 1) : 0) * sizeof(double));
+    total = o;
+  }
+};
+
+// The plan update with each thread's plan elements held in registers from
+// load to store, for s <= 64 and objects of at most NC * (NT / s) supports.
+// Thread tid owns row k = tid % s of columns j = tid / s + c (NT / s),
+// c < NC, so the element offsets k + j s are contiguous over the workgroup.
+// Per object:
+//   loads: all of the thread's Z, Y (and stored costs) are requested at the
+//          top, with the weights and supports, so the object costs one HBM
+//          round trip;
+//   pass 1: t = Z exp(-(C + Y)) + 1e-9 (:155-157), kept, and written to LDS;
+//   pass 2: four threads per column sum its t (_dcnorm, :158) into a column
+//           scale w_j / colsum;
+//   pass 3: x = t * scale (:159), z' = x exp(Y) + 1e-9 (:166), Y + X, and
+//           per-thread partial row sums of z' (_drnorm, :169) and, on support
+//           iterations, of X and X * supports (:206-211);
+//   step 4: wave 0 adds the NT / s partials of each row in a fixed order;
+//   pass 4: z = z' / rowsum * c_w (:169-170), Y += X - Z (:175), stored.
+// Every sum has a fixed order: results are reproducible run to run.
+//
+// MODE 0 (lean): passes after the first with no support update or residual;
+// MODE 1: the first pass and support-update passes; MODE 2: residual passes
+// (anything).  X stays in registers until pass 4 only where pass 4 needs it:
+// residuals, and FP64, whose Y update keeps the reference's Y + (X - Z).
+template <typename T, int CM, int DIM, int NC, int MODE>
+__global__ __launch_bounds__(NT)
+__attribute__((amdgpu_waves_per_eu(
+    MODE == 0 || (MODE == 1 && D2MI_PLANREG_MID_CAP && !std::is_same<T, double>::value)
+        ? D2MI_PLANREG_WAVES_PER_EU(NC)
+        : (MODE == 2 && !std::is_same<T, double>::value ? D2MI_PLANREG_RESID_WAVES : 1))))
+void k_badmm_plan_reg(PlanArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr bool F64 = std::is_same<T, double>::value;
+  constexpr int U = PLAN_UNROLL;
+  typedef typename Vec2<T>::type T2;
+  const int s = a.s, d = DIM > 0 ? DIM : a.d, sp = a.sp, tid = threadIdx.x;
+  const int lane = tid & 63, wv = tid >> 6;
+  const bool upd = MODE >= 1 && a.upd, first = MODE >= 1 && a.first;
+  const bool resid = MODE == 2 && a.resid;
+  const bool keepx = MODE >= 1 && (F64 || resid);  // x held until pass 4
+  const PlanRegLds L(sizeof(T), sp, a.m_max, s, d, a.R);
+  T *A = (T *)(smem + L.A), *csl = (T *)(smem + L.col);
+  T *wl = (T *)(smem + L.w), *xs = (T *)(smem + L.xs);
+  T *cs = (T *)(smem + L.cs), *cwl = (T *)(smem + L.cw), *g = (T *)(smem + L.g);
+  T *rp = (T *)(smem + L.rp);
+  int *ids = (int *)(smem + L.ids);
+  double *acc = (double *)(smem + L.acc), *red = (double *)(smem + L.red);
+  double *up = (double *)(smem + L.up);
+
+  const int gi = blockIdx.x;
+  const int p0 = gi * a.chunk, p1 = min(a.N, p0 + a.chunk);
+  const double rho = a.scal[0];
+  const T inv_rho = (T)a.scal[1];
+  const int cpi = NT / s, ke_t = tid % s;
+  const int jb_t = tid < cpi * s ? tid / s : 0x3fffffff;  // idle threads own no column
+  const int q4 = tid & 3, RQ = (((s + 3) >> 2) + 1) & ~1, r0 = q4 * RQ;
+  const int nr = max(0, min(RQ, s - r0));  // pass 2: rows of this thread in a column
+  double r_obj = 0.0, r_pr = 0.0, r_dr = 0.0;
+  for (int q = tid; q < a.R; q += NT) acc[q] = 0.0;
+  int cur = -1;
+
+  for (int p = p0; p < p1; ++p) {
+    // per-object copies of the thread's row and first column: without this
+    // the compiler keeps every column slot's addresses live across objects
+    int ke = ke_t, jb = jb_t;
+    asm volatile("" : "+v"(ke), "+v"(jb));
+    const int o = const_load(a.perm + p);
+    const int l = const_load(a.label + o);
+    const int m = const_load(a.p_str + o);
+    const long long off = const_load(a.cum + o);
+    const size_t base = (size_t)s * off;
+    const bool reinit = first && (a.force_init || const_load(a.lsw + o) == 1);
+    T *Zo = a.Z + base, *Yo = a.Y + base;  // this object's plan (offsets k + j s)
+    // requests first: plan elements, weights, supports or bin ids
+    T zr[NC], yr[NC], xr[NC], cr[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const int j = jb + c * cpi;
+      zr[c] = (T)0;
+      yr[c] = (T)0;
+      if (j < m) {
+        if (!reinit) zr[c] = Zo[ke + j * s];
+        if (!first) yr[c] = Yo[ke + j * s];
+        if constexpr (CM == COST_TABLE) cr[c] = a.ctab[base + ke + j * s];
+      }
+    }
+    T wpre[PRE_W], xpre[PRE_X];
+    int ipre[PRE_W];
+#pragma unroll
+    for (int r = 0; r < PRE_W; ++r) {
+      const int q = tid + r * NT;
+      if (q < m) {
+        wpre[r] = a.w[off + q];
+        if constexpr (CM == COST_SPARSE) ipre[r] = a.sym[off + q];
+      }
+    }
+    if constexpr (CM == COST_EUCLID) {
+#pragma unroll
+      for (int r = 0; r < PRE_X; ++r) {
+        const int q = tid + r * NT;
+        if (q < m * d) xpre[r] = a.supp[(size_t)off * d + q];
+      }
+    }
+    if (l != cur) {
+      __syncthreads();  // every wave is done with the previous label
+      if (cur >= 0)
+        for (int q = tid; q < a.R; q += NT) {
+          a.rec[(size_t)(gi + cur) * a.R + q] = acc[q];
+          acc[q] = 0.0;
+        }
+      cur = l;
+      for (int q = tid; q < s; q += NT) cwl[q] = a.cw[(size_t)l * s + q];
+      if constexpr (CM == COST_EUCLID)
+        for (int q = tid; q < s * d; q += NT) cs[q] = a.csupp[(size_t)l * s * d + q];
+    }
+#pragma unroll
+    for (int r = 0; r < PRE_W; ++r) {
+      const int q = tid + r * NT;
+      if (q < m) {
+        wl[q] = wpre[r];
+        if constexpr (CM == COST_SPARSE) ids[q] = ipre[r];
+      }
+    }
+    if constexpr (CM == COST_EUCLID) {
+#pragma unroll
+      for (int r = 0; r < PRE_X; ++r) {
+        const int q = tid + r * NT;
+        if (q < m * d) xs[q] = xpre[r];
+      }
+    }
+    __syncthreads();  // also: every wave has finished the previous object's last pass
+    const T cwk = cwl[ke];
+    T csk[DIM > 0 ? DIM : 1];
+    if constexpr (CM == COST_EUCLID && DIM > 0) {
+#pragma unroll
+      for (int dd = 0; dd < DIM; ++dd) csk[dd] = cs[ke * DIM + dd];
+    }
+    // ground cost of (ke, j) before the division by rho (stored costs: cr)
+    auto cost_k = [&](int j) -> T {
+      if constexpr (CM == COST_EUCLID) {
+        T c = 0;
+        if constexpr (DIM > 0) {
+#pragma unroll
+          for (int dd = 0; dd < DIM; ++dd) {
+            const T u = csk[dd] - xs[j * DIM + dd];
+            c += u * u;
+          }
+        } else {
+          for (int dd = 0; dd < d; ++dd) {
+            const T u = cs[ke * d + dd] - xs[j * d + dd];
+            c += u * u;
+          }
+        }
+        return c;
+      } else if constexpr (CM == COST_HIST) {
+        return a.dist[ke + (size_t)j * s];
+      } else if constexpr (CM == COST_SPARSE) {
+        return a.dist[(size_t)ids[j] * a.V + ke];
+      } else {
+        return a.ctab[base + ke + j * s];
+      }
+    };
+
+    // pass 1 -- step 1a: t = Z exp(-(C + Y)) + 1e-9 (:155-157)
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const int j = jb + c * cpi;
+      if (j < m) {
+        const T cost = CM == COST_TABLE ? cr[c] : cost_k(j);
+        const T z = reinit ? wl[j] * cwk : zr[c];
+        const T t = z * plan_exp<T>(-(scale_cost<T>(cost, rho, inv_rho) + yr[c])) + (T)1e-9;
+        A[ke + j * sp] = t;
+        zr[c] = t;
+      }
+    }
+    __syncthreads();
+    // pass 2 -- step 1b: column sums (_dcnorm, :158) -> the column's scale
+    for (int j = tid >> 2; j < m; j += NT / 4) {
+      const T *colA = A + j * sp + r0;
+      T part = 0;
+      for (int r = 0; r < nr; r += 2) {
+        const T2 v = *(const T2 *)(colA + r);
+        part += v.x;
+        if (r + 1 < nr) part += v.y;
+      }
+      part += __shfl_xor(part, 1, 64);
+      part += __shfl_xor(part, 2, 64);
+      if (q4 == 0) csl[j] = F64 ? part : wl[j] / part;
+    }
+    __syncthreads();
+    // pass 3 -- step 1c: x = t / colsum * w (:159); step 2a: z' = x exp(Y) +
+    // 1e-9 (:166); partial row sums (_drnorm, :169) and step 5 partials
+    {
+      T rpart = 0;
+      double P[DIM > 0 ? DIM : MAXD_FUSED], rx = 0.0;
+      if (upd)
+        for (int dd = 0; dd < d; ++dd) P[dd] = 0.0;
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        const int j = jb + c * cpi;
+        if (j < m) {
+          const T x = F64 ? (zr[c] / csl[j]) * wl[j] : zr[c] * csl[j];
+          const T zp = x * plan_exp<T>(yr[c]) + (T)1e-9;
+          rpart += zp;
+          zr[c] = zp;
+          if (keepx) xr[c] = x;
+          else yr[c] += x;
+          if (upd) {
+            rx += (double)x;  // row sums of X (:211)
+            if constexpr (CM == COST_EUCLID)
+              for (int dd = 0; dd < d; ++dd) P[dd] += (double)x * (double)xs[j * d + dd];  // (:206)
+            if constexpr (CM == COST_TABLE) a.xout[base + ke + j * s] = x;  // for the support GEMM
+          }
+        }
+      }
+      if (jb < cpi) {
+        rp[jb * s + ke] = rpart;
+        if (upd) {
+          double *u = up + (size_t)(jb * s + ke) * (d + 1);
+          for (int dd = 0; dd < d; ++dd) u[dd] = P[dd];
+          u[d] = rx;
+        }
+      }
+    }
+    __syncthreads();
+    // step 4: normalised row sums summed per label (:180-185)
+    if (wv == 0) {
+      T r = 0;
+      if (lane < s) {
+        for (int c = 0; c < cpi; ++c) r += rp[c * s + lane];
+        g[lane] = F64 ? r : cwl[lane] / r;
+      }
+      const double rs = wave_sum(lane < s ? (double)r : 0.0);
+      if (lane < s) acc[lane] += (double)r / rs;
+      if (upd && lane < s) {
+        for (int dd = 0; dd <= d; ++dd) {
+          double v = 0.0;
+          for (int c = 0; c < cpi; ++c) v += up[(size_t)(c * s + lane) * (d + 1) + dd];
+          if (dd < d) acc[s + lane * d + dd] += v;
+          else acc[s + s * d + lane] += v;
+        }
+      }
+    }
+    __syncthreads();
+    // pass 4 -- step 2c + 3: z = z' / rowsum * c_w (:169-170); Y += X - Z (:175)
+    {
+      const T gk = g[ke];
+#pragma unroll
+      for (int c0 = 0; c0 < NC; c0 += U) {
+        T z0v[U], cv[U];
+        if (resid) {  // step 6 (:300-305) and the exit objective: Z before this pass
+#pragma unroll
+          for (int u = 0; u < U && c0 + u < NC; ++u) {
+            const int j = jb + (c0 + u) * cpi;
+            if (j < m) {
+              z0v[u] = reinit ? wl[j] * cwk : Zo[ke + j * s];
+              cv[u] = cost_k(j);
+            }
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < U && c0 + u < NC; ++u) {
+          const int c = c0 + u, j = jb + c * cpi;
+          if (j < m) {
+            const int e = ke + j * s;
+            const T z = F64 ? (zr[c] / gk) * cwk : zr[c] * gk;
+            if (resid) {
+              const T cost = scale_cost<T>(cv[u], rho, inv_rho);
+              r_obj += (double)cost * (double)xr[c];
+              r_pr += fabs((double)xr[c] - (double)z);
+              r_dr += fabs((double)z0v[u] - (double)z);
+            }
+            Zo[e] = z;
+            Yo[e] = keepx ? yr[c] + (xr[c] - z) : yr[c] - z;
+          }
+        }
+      }
+    }
+    if (resid) __syncthreads();  // pass 4 read wl / xs: keep them until all waves are done
+  }
+  __syncthreads();
+  if (cur >= 0)
+    for (int q = tid; q < a.R; q += NT) a.rec[(size_t)(gi + cur) * a.R + q] = acc[q];
+  if (resid) {
     const double t0 = block_sum(r_obj, red);
     const double t1 = block_sum(r_pr, red);
     const double t2 = block_sum(r_dr, red);
@@ -1034,36 +1731,90 @@ int d2mi_ensure_plans(d2mi_ctx *ctx, PhaseDev *p, int precision) {
 
 namespace {
 
-// resident workgroups per CU of the plan kernel at this LDS size (HIP's
+// resident workgroups per CU of a plan kernel at this LDS size (HIP's
 // occupancy calculator), so one launch is exactly one round of workgroups
-template <typename T, int CM, int DIM, bool GM = false>
-int plan_occupancy(size_t lds) {
+template <typename F>
+int occupancy_of(F kernel, size_t lds) {
   int n = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_badmm_plan<T, CM, 0, DIM, GM>, NT, lds) !=
-          hipSuccess || n <= 0)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, NT, lds) != hipSuccess || n <= 0)
     n = 1;
   return n;
 }
 
+// the plan kernel for an update: k_badmm_plan_reg (plan elements in
+// registers; s <= 64, objects of up to NC * (NT / s) supports), k_badmm_plan64
+// (plan in LDS, s <= 64), k_badmm_plan (any s) or its HBM-scratch form (GM);
+// LEAN instances serve passes after the first with no support update,
+// residual or kept X
+enum PlanKind { PK_LEGACY, PK_GM, PK_64, PK_REG8, PK_REG16 };
+
 template <typename T, int CM, int DIM>
-void launch_plan(int rmax, int G, size_t lds, hipStream_t S, const PlanArgs<T> &pa, bool gm) {
-  if (gm) {
-    k_badmm_plan<T, CM, 0, DIM, true><<<G, NT, lds, S>>>(pa, pa.perm, pa.p_str, pa.cum, pa.label,
-                                                         pa.lsw);
-    return;
+struct PlanKernels {
+  // lean instances exist for FP32 only (FP64 passes always keep X), and FP64
+  // register forms for up to 8 columns per thread
+  static constexpr bool F32 = !std::is_same<T, double>::value;
+  // register-form mode: 0 lean, 1 first / support-update passes, 2 residual
+  template <bool LEAN>
+  static int reg_mode(const PlanArgs<T> &pa) {
+    return LEAN ? 0 : pa.resid ? 2 : 1;
   }
-#ifndef D2MI_PLAN_EARLY_LOADS
-  // Measured on MI355X: holding a whole object's Z/Y in registers costs more
-  // occupancy than it saves in latency, so by default loads stay in pass 1.
-  rmax = 0;
-#endif
-  switch (rmax) {
-    case 8: k_badmm_plan<T, CM, 8, DIM><<<G, NT, lds, S>>>(pa, pa.perm, pa.p_str, pa.cum, pa.label, pa.lsw); break;
-    case 16: k_badmm_plan<T, CM, 16, DIM><<<G, NT, lds, S>>>(pa, pa.perm, pa.p_str, pa.cum, pa.label, pa.lsw); break;
-    case 32: k_badmm_plan<T, CM, 32, DIM><<<G, NT, lds, S>>>(pa, pa.perm, pa.p_str, pa.cum, pa.label, pa.lsw); break;
-    default: k_badmm_plan<T, CM, 0, DIM><<<G, NT, lds, S>>>(pa, pa.perm, pa.p_str, pa.cum, pa.label, pa.lsw); break;
+  template <bool LEAN>
+  static int occ(size_t lds, PlanKind k, int mode) {
+    switch (k) {
+      case PK_REG16:
+        if constexpr (F32) {
+          if constexpr (LEAN) return occupancy_of(k_badmm_plan_reg<T, CM, DIM, 16, 0>, lds);
+          return mode == 2 ? occupancy_of(k_badmm_plan_reg<T, CM, DIM, 16, 2>, lds)
+                           : occupancy_of(k_badmm_plan_reg<T, CM, DIM, 16, 1>, lds);
+        }
+        return 0;
+      case PK_REG8:
+        if constexpr (LEAN) return occupancy_of(k_badmm_plan_reg<T, CM, DIM, 8, 0>, lds);
+        return mode == 2 ? occupancy_of(k_badmm_plan_reg<T, CM, DIM, 8, 2>, lds)
+                         : occupancy_of(k_badmm_plan_reg<T, CM, DIM, 8, 1>, lds);
+      case PK_64: return occupancy_of(k_badmm_plan64<T, CM, DIM, LEAN>, lds);
+      case PK_GM: return occupancy_of(k_badmm_plan<T, CM, 0, true>, lds);
+      default: return occupancy_of(k_badmm_plan<T, CM, DIM, false, LEAN>, lds);
+    }
   }
-}
+  template <bool LEAN>
+  static void run(int G, size_t lds, hipStream_t S, const PlanArgs<T> &pa, PlanKind k) {
+    const int mode = reg_mode<LEAN>(pa);
+    switch (k) {
+      case PK_REG16:
+        if constexpr (F32) {
+          if constexpr (LEAN) k_badmm_plan_reg<T, CM, DIM, 16, 0><<<G, NT, lds, S>>>(pa);
+          else if (mode == 2) k_badmm_plan_reg<T, CM, DIM, 16, 2><<<G, NT, lds, S>>>(pa);
+          else k_badmm_plan_reg<T, CM, DIM, 16, 1><<<G, NT, lds, S>>>(pa);
+        }
+        break;
+      case PK_REG8:
+        if constexpr (LEAN) k_badmm_plan_reg<T, CM, DIM, 8, 0><<<G, NT, lds, S>>>(pa);
+        else if (mode == 2) k_badmm_plan_reg<T, CM, DIM, 8, 2><<<G, NT, lds, S>>>(pa);
+        else k_badmm_plan_reg<T, CM, DIM, 8, 1><<<G, NT, lds, S>>>(pa);
+        break;
+      case PK_64: k_badmm_plan64<T, CM, DIM, LEAN><<<G, NT, lds, S>>>(pa); break;
+      case PK_GM:
+        k_badmm_plan<T, CM, 0, true><<<G, NT, lds, S>>>(pa, pa.perm, pa.p_str, pa.cum, pa.label,
+                                                        pa.lsw);
+        break;
+      default:
+        k_badmm_plan<T, CM, DIM, false, LEAN><<<G, NT, lds, S>>>(pa, pa.perm, pa.p_str, pa.cum,
+                                                                 pa.label, pa.lsw);
+    }
+  }
+  static int occupancy(size_t lds, PlanKind k, bool lean, bool resid) {
+    if constexpr (F32)
+      if (lean && k != PK_GM) return occ<true>(lds, k, 0);
+    return occ<false>(lds, k, resid ? 2 : 1);
+  }
+  static void launch(int G, size_t lds, hipStream_t S, const PlanArgs<T> &pa, PlanKind k,
+                     bool lean) {
+    if constexpr (F32)
+      if (lean && k != PK_GM) return run<true>(G, lds, S, pa, k);
+    run<false>(G, lds, S, pa, k);
+  }
+};
 
 template <typename T, int CM>
 int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int updatePerLoops,
@@ -1204,35 +1955,65 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
   double ms_plan = 0.0;
 
   if (variant == D2MI_BADMM_ENGINE) {
-    const int sp = (s % 2 == 0) ? s + 1 : s;  // odd LDS column stride: conflict-free sums
     const int Rmax = euclid_upd ? s * (d + 2) : s;
+    // kernel choice: k_badmm_plan64 for s <= 64 with the plan in LDS
+    // (D2MI_PLAN_LEGACY=1 forces k_badmm_plan), k_badmm_plan otherwise, in
+    // its HBM-scratch form when an object's plan does not fit LDS
+    static const int pad64 = [] {
+      const char *e = getenv("D2MI_PLAN64_PAD");  // extra column padding (tuning)
+      return e ? 2 * (atoi(e) / 2) : 0;
+    }();
+    const int sp_legacy = (s % 2 == 0) ? s + 1 : s;  // odd column stride: conflict-free sums
+    const int sp64 = s + (s & 1) + pad64;             // even: 2-row LDS words
+    const bool force_gm = getenv_flag("D2MI_BADMM_HBM_SCRATCH");
+    const bool k64 = s <= 64 && !force_gm && !getenv_flag("D2MI_PLAN_LEGACY") &&
+                     Plan64Lds(sizeof(T), sp64, m_max, s, d, 1, Rmax).total <= ctx->lds_max;
+    const bool gm =
+        !k64 && (force_gm || PlanLds(sizeof(T), sp_legacy, m_max, s, d, 1, Rmax).total > ctx->lds_max);
+    // register form when every thread's columns fit its register slots
+    const int ncols = s <= 64 ? (m_max + NT / s - 1) / (NT / s) : 1 << 30;
+    PlanKind kind = gm ? PK_GM : k64 ? PK_64 : PK_LEGACY;
+    if (k64 && !getenv_flag("D2MI_PLAN_NOREG") &&
+        PlanRegLds(sizeof(T), sp64, m_max, s, d, Rmax).total <= ctx->lds_max) {
+      if (ncols <= 8) kind = PK_REG8;
+      else if (!F64 && ncols <= 16) kind = PK_REG16;
+    }
+    const int sp = k64 ? sp64 : sp_legacy;
+    auto lds_of = [&](int keepx, int R) -> size_t {
+      if (kind == PK_REG8 || kind == PK_REG16) return PlanRegLds(sizeof(T), sp, m_max, s, d, R).total;
+      return k64 ? Plan64Lds(sizeof(T), sp, m_max, s, d, keepx, R).total
+                 : PlanLds(sizeof(T), sp, m_max, s, d, keepx, R, gm).total;
+    };
+    // fixed Euclidean dimensions get straight-line cost code
+    const int dim_sel = CM == COST_EUCLID && (d == 2 || d == 3) ? d : 0;
+    auto occupancy = [&](size_t lds, bool lean, bool resid) {
+      if constexpr (CM == COST_EUCLID) {
+        if (dim_sel == 2) return PlanKernels<T, CM, 2>::occupancy(lds, kind, lean, resid);
+        if (dim_sel == 3) return PlanKernels<T, CM, 3>::occupancy(lds, kind, lean, resid);
+      }
+      return PlanKernels<T, CM, 0>::occupancy(lds, kind, lean, resid);
+    };
     // launch geometry: contiguous chunks of the label-sorted order, one chunk
     // per resident workgroup (D2MI_PLAN_WG_PER_CU overrides, for tuning)
     static const int wg_env = [] {
       const char *e = getenv("D2MI_PLAN_WG_PER_CU");
       return e ? atoi(e) : 0;
     }();
-    auto geometry = [&](size_t lds, int *G_out, int *chunk_out, bool g = false) {
-      int per_cu;
-      if (wg_env > 0) per_cu = wg_env;
-      else if (g) per_cu = plan_occupancy<T, CM, 0, true>(lds);
-      else if (CM == COST_EUCLID && d == 2) per_cu = plan_occupancy<T, CM, 2>(lds);
-      else if (CM == COST_EUCLID && d == 3) per_cu = plan_occupancy<T, CM, 3>(lds);
-      else per_cu = plan_occupancy<T, CM, 0>(lds);
+    auto geometry = [&](size_t lds, int *G_out, int *chunk_out, bool lean, bool resid) {
+      const int per_cu = wg_env > 0 ? wg_env : occupancy(lds, lean, resid);
       const int Gt = std::max(1, ctx->num_cu * per_cu);
       *chunk_out = std::max(1, (N + Gt - 1) / Gt);
       *G_out = (N + *chunk_out - 1) / *chunk_out;
     };
-    // objects whose plan does not fit LDS run the HBM-scratch form
-    const size_t lds_fused = PlanLds(sizeof(T), sp, m_max, s, d, 1, Rmax).total;
-    const bool gm = lds_fused > ctx->lds_max || getenv_flag("D2MI_BADMM_HBM_SCRATCH");
     if (gm) {
       D2MI_TRY(p->Xout.ensure(sizeof(T) * (size_t)s * p->col));
       D2MI_TRY(p->Z0.ensure(sizeof(T) * (size_t)s * p->col));
       xout = p->Xout.as<T>();
     }
+    // lean passes: FP32 passes with no support update, residual or kept X
+    const bool lean_ok = !F64 && !gm && !getenv_flag("D2MI_PLAN_NO_LEAN");
     int Gmax = 0, chunk_tmp = 0;
-    geometry(PlanLds(sizeof(T), sp, m_max, s, d, 0, s, gm).total, &Gmax, &chunk_tmp, gm);
+    geometry(lds_of(0, s), &Gmax, &chunk_tmp, lean_ok, false);
     D2MI_TRY(ctx->rec.ensure(sizeof(double) * (size_t)(Gmax + K) * Rmax));
     D2MI_TRY(ctx->part.ensure(sizeof(double) * (size_t)std::max(setup_blocks, 3 * Gmax)));
     if (m_max > NT * PRE_W || (size_t)m_max * std::max(d, 1) > (size_t)NT * PRE_X) {
@@ -1240,9 +2021,7 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
                      "(m <= %d, m*d <= %d)", m_max, d, NT * PRE_W, NT * PRE_X);
       return D2MI_EUNSUPPORTED;
     }
-    const int slots = (s * m_max + NT - 1) / NT;
-    const int rmax = slots <= 8 ? 8 : slots <= 16 ? 16 : slots <= 32 ? 32 : 0;
-    const size_t lds_full = PlanLds(sizeof(T), sp, m_max, s, d, 1, Rmax, gm).total;
+    const size_t lds_full = lds_of(1, Rmax);
     if (lds_full > ctx->lds_max) {
       d2mi_set_error("B-ADMM with s = %d, d = %d needs %zu B of LDS (limit %zu)", s, d, lds_full,
                      ctx->lds_max);
@@ -1255,9 +2034,10 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
       const int resid = is_resid(iter);
       const int keepx = F64 || upd || resid;
       const int R = upd ? s * (d + 2) : s;
-      const size_t lds = PlanLds(sizeof(T), sp, m_max, s, d, keepx, R, gm).total;
+      const size_t lds = lds_of(keepx, R);
+      const bool lean = lean_ok && iter > 0 && !upd && !resid && !keepx;
       int G, chunk;
-      geometry(lds, &G, &chunk, gm);
+      geometry(lds, &G, &chunk, lean, resid);
       if (G > Gmax) {  // records are sized for Gmax workgroups
         chunk = std::max(1, (N + Gmax - 1) / Gmax);
         G = (N + chunk - 1) / chunk;
@@ -1274,12 +2054,11 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
       pa.first = iter == 0; pa.force_init = fresh ? 1 : 0; pa.lsw = ctx->lsw.as<char>();
       D2MI_HIP(hipEventRecord(evs[2 * iter], S));
       if constexpr (CM == COST_EUCLID) {
-        if (gm) launch_plan<T, CM, 0>(rmax, G, lds, S, pa, true);
-        else if (d == 2) launch_plan<T, CM, 2>(rmax, G, lds, S, pa, false);
-        else if (d == 3) launch_plan<T, CM, 3>(rmax, G, lds, S, pa, false);
-        else launch_plan<T, CM, 0>(rmax, G, lds, S, pa, false);
+        if (dim_sel == 2) PlanKernels<T, CM, 2>::launch(G, lds, S, pa, kind, lean);
+        else if (dim_sel == 3) PlanKernels<T, CM, 3>::launch(G, lds, S, pa, kind, lean);
+        else PlanKernels<T, CM, 0>::launch(G, lds, S, pa, kind, lean);
       } else {
-        launch_plan<T, CM, 0>(rmax, G, lds, S, pa, gm);
+        PlanKernels<T, CM, 0>::launch(G, lds, S, pa, kind, lean);
       }
       D2MI_HIP(hipEventRecord(evs[2 * iter + 1], S));
       FinArgs<T> fa{};
