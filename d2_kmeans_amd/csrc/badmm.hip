@@ -837,6 +837,32 @@ __device__ __forceinline__ T const_load(const T *p) {
   return *(const __attribute__((address_space(4))) T *)p;
 }
 
+// Plan I/O of one object through a raw buffer resource on its base: every
+// access is base + voff (the thread's first element, one VGPR) + soff (the
+// column step, an SGPR), so no per-access address registers are live while
+// a store is in flight.  Resource word 3 selects 32-bit data (gfx9 layout);
+// the record count is left open: callers mask their lanes.
+typedef __amdgpu_buffer_rsrc_t BufRes;
+__device__ __forceinline__ BufRes buf_res(const void *base) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), 0, 0x7fffffff, 0x00020000);
+}
+template <typename T>
+__device__ __forceinline__ T buf_load(BufRes r, int voff, int soff) {
+  if constexpr (std::is_same<T, double>::value)
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+  else
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+template <typename T>
+__device__ __forceinline__ void buf_store(T v, BufRes r, int voff, int soff) {
+  if constexpr (std::is_same<T, double>::value) {
+    typedef unsigned int u2 __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, voff, soff, 0);
+  } else {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, voff, soff, 0);
+  }
+}
+
 // LDS layout of k_badmm_plan_reg: only t goes through LDS (for the column
 // sums); column stride sp even, as for k_badmm_plan64.
 struct PlanRegLds {
@@ -931,7 +957,10 @@ void k_badmm_plan_reg(PlanArgs<T> a) {
     const long long off = const_load(a.cum + o);
     const size_t base = (size_t)s * off;
     const bool reinit = first && (a.force_init || const_load(a.lsw + o) == 1);
-    T *Zo = a.Z + base, *Yo = a.Y + base;  // this object's plan (offsets k + j s)
+    // this object's plan: element (k, j) at base + k + j s; the thread's
+    // column slot c is voff + c * cstride bytes from the base
+    const BufRes rz = buf_res(a.Z + base), ry = buf_res(a.Y + base);
+    const int voff = (ke + jb * s) * (int)sizeof(T), cstride = cpi * s * (int)sizeof(T);
     // requests first: plan elements, weights, supports or bin ids
     T zr[NC], yr[NC], xr[NC], cr[NC];
 #pragma unroll
@@ -940,8 +969,8 @@ void k_badmm_plan_reg(PlanArgs<T> a) {
       zr[c] = (T)0;
       yr[c] = (T)0;
       if (j < m) {
-        if (!reinit) zr[c] = Zo[ke + j * s];
-        if (!first) yr[c] = Yo[ke + j * s];
+        if (!reinit) zr[c] = buf_load<T>(rz, voff, c * cstride);
+        if (!first) yr[c] = buf_load<T>(ry, voff, c * cstride);
         if constexpr (CM == COST_TABLE) cr[c] = a.ctab[base + ke + j * s];
       }
     }
@@ -1114,7 +1143,7 @@ void k_badmm_plan_reg(PlanArgs<T> a) {
           for (int u = 0; u < U && c0 + u < NC; ++u) {
             const int j = jb + (c0 + u) * cpi;
             if (j < m) {
-              z0v[u] = reinit ? wl[j] * cwk : Zo[ke + j * s];
+              z0v[u] = reinit ? wl[j] * cwk : buf_load<T>(rz, voff, (c0 + u) * cstride);
               cv[u] = cost_k(j);
             }
           }
@@ -1123,7 +1152,6 @@ void k_badmm_plan_reg(PlanArgs<T> a) {
         for (int u = 0; u < U && c0 + u < NC; ++u) {
           const int c = c0 + u, j = jb + c * cpi;
           if (j < m) {
-            const int e = ke + j * s;
             const T z = F64 ? (zr[c] / gk) * cwk : zr[c] * gk;
             if (resid) {
               const T cost = scale_cost<T>(cv[u], rho, inv_rho);
@@ -1131,9 +1159,21 @@ void k_badmm_plan_reg(PlanArgs<T> a) {
               r_pr += fabs((double)xr[c] - (double)z);
               r_dr += fabs((double)z0v[u] - (double)z);
             }
-            Zo[e] = z;
-            Yo[e] = keepx ? yr[c] + (xr[c] - z) : yr[c] - z;
+            yr[c] = keepx ? yr[c] + (xr[c] - z) : yr[c] - z;
+            zr[c] = z;
           }
+        }
+      }
+      // the stores, from registers that stay untouched while they drain:
+      // every value is materialised first (an instruction that rewrites a
+      // register an in-flight store reads must wait for that store)
+#pragma unroll
+      for (int c = 0; c < NC; ++c) asm volatile("" : "+v"(zr[c]), "+v"(yr[c]));
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        if (jb + c * cpi < m) {
+          buf_store<T>(zr[c], rz, voff, c * cstride);
+          buf_store<T>(yr[c], ry, voff, c * cstride);
         }
       }
     }
