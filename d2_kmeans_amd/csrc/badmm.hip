@@ -1442,6 +1442,140 @@ __global__ __launch_bounds__(NT) void k_badmm_supp_num(SuppArgs<T> a) {
   }
 }
 
+// Step 5 numerators on the matrix cores (FP32): per label, the split-K GEMM
+// sum_i X_i (s x m_i) W_i (m_i x d), W_i the object's supports or word
+// vectors (centroid_Bregman.c:199-212, :229-247).  A workgroup walks its chunk
+// of the label-sorted order in blocks of SM_JB columns of one object (zero
+// past m); wave w owns output column blocks w, w + 4, w + 8 of 32 (d <= 384
+// per sweep) and both 32-row blocks (s <= 64), accumulating in FP32 MFMA
+// tiles; a label change flushes the tiles into the FP64 record (g, l).  The
+// next block's X and W are loaded into registers while the current one is
+// multiplied.  MFMA 32x32x2 f32 operands: lane l holds A[l % 32][l / 32] and
+// B[l / 32][l % 32]; D[i][j] sits in lane j + 32 ((i / 4) % 2), register
+// (i % 4) + 4 (i / 8).
+constexpr int SM_JB = 16;            // columns per block
+constexpr int SM_XP = SM_JB + 1;     // LDS row stride of the X block
+constexpr int SM_TD = 384;           // output columns per sweep (3 blocks of 32 per wave)
+constexpr int SM_WP = SM_TD + 4;     // LDS row stride of the W block
+
+template <bool WORD>
+__global__ __launch_bounds__(NT) void k_badmm_supp_mfma(SuppArgs<float> a) {
+  __shared__ float Xl[64 * SM_XP];
+  __shared__ float Wl[SM_JB * SM_WP];
+  typedef float v16f __attribute__((ext_vector_type(16)));
+  const int s = a.s, d = a.d, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int g = blockIdx.x;
+  const int p0 = g * a.chunk, p1 = min(a.N, p0 + a.chunk);
+  const size_t R = (size_t)s * d;
+  for (int t0 = 0; t0 < d; t0 += SM_TD) {
+    v16f acc[2][3];
+    auto zero = [&]() {
+#pragma unroll
+      for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+        for (int cb = 0; cb < 3; ++cb)
+#pragma unroll
+          for (int r = 0; r < 16; ++r) acc[rb][cb][r] = 0.f;
+    };
+    // record (g, l) as a buffer: lane offset in a VGPR, tile offsets in SGPRs
+    auto flush = [&](int l) {
+      const BufRes rr = buf_res(a.rec + (size_t)(g + l) * R);
+      const int kl = 4 * (lane >> 5), tl = t0 + wv * 32 + (lane & 31);
+      const int voff = (kl * d + tl) * (int)sizeof(double);
+#pragma unroll
+      for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+        for (int cb = 0; cb < 3; ++cb)
+#pragma unroll
+          for (int r = 0; r < 16; ++r) {
+            const int kr = rb * 32 + (r & 3) + 8 * (r >> 2);  // row less the lane part
+            if (kr + kl < s && tl + 128 * cb < d)
+              buf_store<double>((double)acc[rb][cb][r], rr, voff,
+                                (kr * d + 128 * cb) * (int)sizeof(double));
+          }
+    };
+    // register prefetch of one block: lane k (row) of X columns
+    // j = wave + 4 u, and W rows j = wave + 4 u at dimensions t0 + lane + 64 q
+    constexpr int XU = SM_JB / 4, WQ = SM_TD / 64;
+    float xr[XU], wr[XU][WQ];
+    auto fetch = [&](int o, int j0) {
+      const int m = const_load(a.p_str + o);
+      const long long off = const_load(a.cum + o);
+      int ln = lane;
+      asm volatile("" : "+v"(ln));  // keep per-lane offsets from being hoisted
+      const float *Xo = a.X + (size_t)s * off;
+#pragma unroll
+      for (int u = 0; u < XU; ++u) {
+        const int j = j0 + wv + 4 * u;
+        xr[u] = (ln < s && j < m) ? Xo[(size_t)j * s + ln] : 0.f;
+      }
+#pragma unroll
+      for (int u = 0; u < XU; ++u) {
+        const int j = j0 + wv + 4 * u;
+        const bool live = j < m;
+        const float *w = a.vocab;
+        if (live) {
+          if constexpr (WORD) w = a.vocab + (size_t)const_load(a.sym + off + j) * d;
+          else w = a.supp + (size_t)(off + j) * d;
+        }
+#pragma unroll
+        for (int q = 0; q < WQ; ++q) {
+          const int t = t0 + ln + 64 * q;
+          wr[u][q] = (live && t < d) ? w[t] : 0.f;
+        }
+      }
+    };
+    auto stage = [&]() {
+#pragma unroll
+      for (int u = 0; u < XU; ++u) Xl[lane * SM_XP + wv + 4 * u] = xr[u];
+#pragma unroll
+      for (int u = 0; u < XU; ++u)
+#pragma unroll
+        for (int q = 0; q < WQ; ++q) Wl[(wv + 4 * u) * SM_WP + lane + 64 * q] = wr[u][q];
+    };
+    // the chunk's blocks in order: (object p, first column j0)
+    int p = p0, j0 = 0, cur = -1;
+    if (p < p1) fetch(const_load(a.perm + p), 0);
+    zero();
+    while (p < p1) {
+      const int o = const_load(a.perm + p), l = const_load(a.label + o), m = const_load(a.p_str + o);
+      if (l != cur) {
+        if (cur >= 0) flush(cur);
+        zero();
+        cur = l;
+      }
+      __syncthreads();  // the previous block is consumed
+      stage();
+      __syncthreads();
+      // advance, and request the next block before multiplying this one
+      int pn = p, jn = j0 + SM_JB;
+      if (jn >= m) {
+        ++pn;
+        jn = 0;
+      }
+      if (pn < p1) fetch(const_load(a.perm + pn), jn);
+#pragma unroll
+      for (int kk = 0; kk < SM_JB; kk += 2) {
+        const int kq = kk + (lane >> 5);
+        float av[2];
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb) av[rb] = Xl[(rb * 32 + (lane & 31)) * SM_XP + kq];
+#pragma unroll
+        for (int cb = 0; cb < 3; ++cb) {
+          const float bv = Wl[kq * SM_WP + (wv + 4 * cb) * 32 + (lane & 31)];
+#pragma unroll
+          for (int rb = 0; rb < 2; ++rb)
+            acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[rb], bv, acc[rb][cb], 0, 0, 0);
+        }
+      }
+      p = pn;
+      j0 = jn;
+    }
+    if (cur >= 0) flush(cur);
+    __syncthreads();  // LDS reuse by the next sweep
+  }
+}
+
 // new supports = numerators / X row sums (_dirms, :219-221), records summed
 // in workgroup order
 template <typename T>
@@ -1922,8 +2056,21 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
     sa2.p_str = p->p_str.as<int>(); sa2.cum = p->cum.as<long long>(); sa2.label = ctx->label.as<int>();
     sa2.supp = supp; sa2.sym = sym; sa2.vocab = vocab; sa2.X = xout; sa2.rec = ctx->rec2.as<double>();
     const int G = (N + tab_chunk - 1) / tab_chunk;
-    if (word) k_badmm_supp_num<T, true><<<G, NT, 0, S>>>(sa2);
-    else k_badmm_supp_num<T, false><<<G, NT, 0, S>>>(sa2);
+    // FP32: matrix cores; FP64 (and s > 64): FP64 FMAs
+    if constexpr (!F64) {
+      if (s <= 64 && !getenv_flag("D2MI_SUPP_VALU")) {
+        if (word) k_badmm_supp_mfma<true><<<G, NT, 0, S>>>(sa2);
+        else k_badmm_supp_mfma<false><<<G, NT, 0, S>>>(sa2);
+      } else if (word) {
+        k_badmm_supp_num<T, true><<<G, NT, 0, S>>>(sa2);
+      } else {
+        k_badmm_supp_num<T, false><<<G, NT, 0, S>>>(sa2);
+      }
+    } else if (word) {
+      k_badmm_supp_num<T, true><<<G, NT, 0, S>>>(sa2);
+    } else {
+      k_badmm_supp_num<T, false><<<G, NT, 0, S>>>(sa2);
+    }
     k_badmm_supp_fin<T><<<K, NT, 0, S>>>(s, dsup, tab_chunk, seg, ctx->rec2.as<double>(),
                                          ctx->rxbuf.as<double>(), p->cs64.as<double>(), csT, flags);
     D2MI_HIP(hipGetLastError());
