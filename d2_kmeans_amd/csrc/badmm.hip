@@ -1196,10 +1196,18 @@ void k_badmm_plan_reg(PlanArgs<T> a) {
 
 // ------------------------------------------------------------- finalize
 
+constexpr int MAXB = 8;  // object-size buckets of one update
+
 template <typename T>
 struct FinArgs {
-  int K, s, d, chunk, R, G, upd, resid, iter;
-  const int *__restrict__ seg;
+  int K, s, d, R, G, upd, resid, iter;
+  // records of bucket b: objects [start[b], start[b + 1]) of the order by
+  // (bucket, label), chunks of chunk[b], record (g, l) at rec_off[b] + g + l;
+  // label l of bucket b spans segk[b K + l] .. segk[b K + l + 1]
+  int nbk;
+  int chunk[MAXB], start[MAXB];
+  long long rec_off[MAXB];
+  const int *__restrict__ segk;
   const double *__restrict__ rec;
   const double *__restrict__ resid_part;
   double *__restrict__ cw64;
@@ -1217,11 +1225,14 @@ __global__ __launch_bounds__(NT) void k_badmm_finalize(FinArgs<T> a) {
   double *sums = (double *)smem;
   double *red = sums + a.R;
   const int l = blockIdx.x, tid = threadIdx.x, s = a.s, d = a.d;
-  const int lo = a.seg[l], hi = a.seg[l + 1];
-  const int glo = lo / a.chunk, ghi = (hi - 1) / a.chunk;
   for (int q = tid; q < a.R; q += NT) {
     double v = 0.0;
-    for (int g = glo; g <= ghi; ++g) v += a.rec[(size_t)(g + l) * a.R + q];
+    for (int b = 0; b < a.nbk; ++b) {  // buckets, then workgroups, in order
+      const int lo = a.segk[b * a.K + l] - a.start[b], hi = a.segk[b * a.K + l + 1] - a.start[b];
+      if (hi <= lo) continue;
+      const int glo = lo / a.chunk[b], ghi = (hi - 1) / a.chunk[b];
+      for (int g = glo; g <= ghi; ++g) v += a.rec[(size_t)(a.rec_off[b] + g + l) * a.R + q];
+    }
     sums[q] = v;
   }
   __syncthreads();
@@ -1752,6 +1763,22 @@ __global__ __launch_bounds__(NT) void k_label_scatter(const int *__restrict__ la
   }
 }
 
+// sort key of the per-bucket order: bucket(m) * K + label, bucket b the
+// first with m <= lim[b]
+struct BucketLims {
+  int n, lim[MAXB];
+};
+__global__ __launch_bounds__(NT) void k_bucket_key(const int *__restrict__ label,
+                                                   const int *__restrict__ p_str, int N, int K,
+                                                   BucketLims bl, int *__restrict__ key) {
+  const int o = blockIdx.x * NT + threadIdx.x;
+  if (o >= N) return;
+  const int m = p_str[o];
+  int b = 0;
+  while (b + 1 < bl.n && m > bl.lim[b]) ++b;
+  key[o] = b * K + label[o];
+}
+
 // ------------------------------------------ plain one-thread-per-object form
 
 template <typename T>
@@ -2143,77 +2170,150 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
 
   if (variant == D2MI_BADMM_ENGINE) {
     const int Rmax = euclid_upd ? s * (d + 2) : s;
-    // kernel choice: k_badmm_plan64 for s <= 64 with the plan in LDS
-    // (D2MI_PLAN_LEGACY=1 forces k_badmm_plan), k_badmm_plan otherwise, in
-    // its HBM-scratch form when an object's plan does not fit LDS
-    static const int pad64 = [] {
-      const char *e = getenv("D2MI_PLAN64_PAD");  // extra column padding (tuning)
-      return e ? 2 * (atoi(e) / 2) : 0;
-    }();
-    const int sp_legacy = (s % 2 == 0) ? s + 1 : s;  // odd column stride: conflict-free sums
-    const int sp64 = s + (s & 1) + pad64;             // even: 2-row LDS words
-    const bool force_gm = getenv_flag("D2MI_BADMM_HBM_SCRATCH");
-    const bool k64 = s <= 64 && !force_gm && !getenv_flag("D2MI_PLAN_LEGACY") &&
-                     Plan64Lds(sizeof(T), sp64, m_max, s, d, 1, Rmax).total <= ctx->lds_max;
-    const bool gm =
-        !k64 && (force_gm || PlanLds(sizeof(T), sp_legacy, m_max, s, d, 1, Rmax).total > ctx->lds_max);
-    // register form when every thread's columns fit its register slots
-    const int ncols = s <= 64 ? (m_max + NT / s - 1) / (NT / s) : 1 << 30;
-    PlanKind kind = gm ? PK_GM : k64 ? PK_64 : PK_LEGACY;
-    if (k64 && !getenv_flag("D2MI_PLAN_NOREG") &&
-        PlanRegLds(sizeof(T), sp64, m_max, s, d, Rmax).total <= ctx->lds_max) {
-      if (ncols <= 8) kind = PK_REG8;
-      else if (!F64 && ncols <= 16) kind = PK_REG16;
-    }
-    const int sp = k64 ? sp64 : sp_legacy;
-    auto lds_of = [&](int keepx, int R) -> size_t {
-      if (kind == PK_REG8 || kind == PK_REG16) return PlanRegLds(sizeof(T), sp, m_max, s, d, R).total;
-      return k64 ? Plan64Lds(sizeof(T), sp, m_max, s, d, keepx, R).total
-                 : PlanLds(sizeof(T), sp, m_max, s, d, keepx, R, gm).total;
-    };
-    // fixed Euclidean dimensions get straight-line cost code
-    const int dim_sel = CM == COST_EUCLID && (d == 2 || d == 3) ? d : 0;
-    auto occupancy = [&](size_t lds, bool lean, bool resid) {
-      if constexpr (CM == COST_EUCLID) {
-        if (dim_sel == 2) return PlanKernels<T, CM, 2>::occupancy(lds, kind, lean, resid);
-        if (dim_sel == 3) return PlanKernels<T, CM, 3>::occupancy(lds, kind, lean, resid);
-      }
-      return PlanKernels<T, CM, 0>::occupancy(lds, kind, lean, resid);
-    };
-    // launch geometry: contiguous chunks of the label-sorted order, one chunk
-    // per resident workgroup (D2MI_PLAN_WG_PER_CU overrides, for tuning)
-    static const int wg_env = [] {
-      const char *e = getenv("D2MI_PLAN_WG_PER_CU");
-      return e ? atoi(e) : 0;
-    }();
-    auto geometry = [&](size_t lds, int *G_out, int *chunk_out, bool lean, bool resid) {
-      const int per_cu = wg_env > 0 ? wg_env : occupancy(lds, lean, resid);
-      const int Gt = std::max(1, ctx->num_cu * per_cu);
-      *chunk_out = std::max(1, (N + Gt - 1) / Gt);
-      *G_out = (N + *chunk_out - 1) / *chunk_out;
-    };
-    if (gm) {
-      D2MI_TRY(p->Xout.ensure(sizeof(T) * (size_t)s * p->col));
-      D2MI_TRY(p->Z0.ensure(sizeof(T) * (size_t)s * p->col));
-      xout = p->Xout.as<T>();
-    }
-    // lean passes: FP32 passes with no support update, residual or kept X
-    const bool lean_ok = !F64 && !gm && !getenv_flag("D2MI_PLAN_NO_LEAN");
-    int Gmax = 0, chunk_tmp = 0;
-    geometry(lds_of(0, s), &Gmax, &chunk_tmp, lean_ok, false);
-    D2MI_TRY(ctx->rec.ensure(sizeof(double) * (size_t)(Gmax + K) * Rmax));
-    D2MI_TRY(ctx->part.ensure(sizeof(double) * (size_t)std::max(setup_blocks, 3 * Gmax)));
     if (m_max > NT * PRE_W || (size_t)m_max * std::max(d, 1) > (size_t)NT * PRE_X) {
       d2mi_set_error("objects of up to %d supports (d=%d) exceed this build's plan kernel "
                      "(m <= %d, m*d <= %d)", m_max, d, NT * PRE_W, NT * PRE_X);
       return D2MI_EUNSUPPORTED;
     }
-    const size_t lds_full = lds_of(1, Rmax);
-    if (lds_full > ctx->lds_max) {
-      d2mi_set_error("B-ADMM with s = %d, d = %d needs %zu B of LDS (limit %zu)", s, d, lds_full,
-                     ctx->lds_max);
-      return D2MI_EUNSUPPORTED;
+    static const int pad64 = [] {
+      const char *e = getenv("D2MI_PLAN64_PAD");  // extra column padding (tuning)
+      return e ? 2 * (atoi(e) / 2) : 0;
+    }();
+    // launch geometry: contiguous chunks of a bucket's label-sorted order,
+    // one chunk per resident workgroup (D2MI_PLAN_WG_PER_CU overrides)
+    static const int wg_env = [] {
+      const char *e = getenv("D2MI_PLAN_WG_PER_CU");
+      return e ? atoi(e) : 0;
+    }();
+    const bool force_gm = getenv_flag("D2MI_BADMM_HBM_SCRATCH");
+    const bool no_64 = getenv_flag("D2MI_PLAN_LEGACY"), no_reg = getenv_flag("D2MI_PLAN_NOREG");
+    const bool no_lean = getenv_flag("D2MI_PLAN_NO_LEAN");
+    // fixed Euclidean dimensions get straight-line cost code
+    const int dim_sel = CM == COST_EUCLID && (d == 2 || d == 3) ? d : 0;
+
+    // ---- object-size buckets: when the largest object is too big for the
+    // register form, objects are launched by size class, each class with the
+    // kernel form and LDS its own largest object allows. For s <= 64 the
+    // classes end at the register form's limit (16 columns per thread in
+    // FP32, 8 in FP64), then twice, four and eight times that, then m_max.
+    // D2MI_PLAN_NO_BUCKETS=1 launches all objects together.
+    std::vector<int> lims;
+    if (s <= 64 && !getenv_flag("D2MI_PLAN_NO_BUCKETS")) {
+      const int cpi = NT / s;
+      for (int c = F64 ? 8 : 16; c * cpi < m_max && (int)lims.size() < 4; c *= 2)
+        lims.push_back(c * cpi);
     }
+    lims.push_back(m_max);
+    const int nbk = (int)lims.size();
+    struct Bucket {
+      int m_max, start, count, sp, Gmax;
+      bool k64, gm;
+      PlanKind kind;
+      long long rec_off;
+    };
+    std::vector<Bucket> bk(nbk);
+    const int *permk = ctx->perm.as<int>(), *segk = seg;
+    std::vector<int> hsegk;
+    if (nbk > 1) {
+      const int Kb = nbk * K;
+      D2MI_TRY(ctx->bkey.ensure(sizeof(int) * (size_t)N));
+      D2MI_TRY(ctx->permk.ensure(sizeof(int) * (size_t)N));
+      D2MI_TRY(ctx->blkcnt2.ensure(sizeof(int) * (size_t)nb * Kb));
+      D2MI_TRY(ctx->segk.ensure(sizeof(int) * (size_t)(2 * Kb + 2)));
+      BucketLims bl{};
+      bl.n = nbk;
+      for (int b = 0; b < nbk; ++b) bl.lim[b] = lims[b];
+      int *sk = ctx->segk.as<int>(), *tot2 = sk + Kb + 1;
+      k_bucket_key<<<nb, NT, 0, S>>>(ctx->label.as<int>(), p->p_str.as<int>(), N, K, bl,
+                                     ctx->bkey.as<int>());
+      k_label_hist<<<nb, NT, sizeof(int) * Kb, S>>>(ctx->bkey.as<int>(), N, Kb,
+                                                    ctx->blkcnt2.as<int>(), flags + 2);
+      k_label_scan<<<Kb, NT, 0, S>>>(nb, Kb, ctx->blkcnt2.as<int>(), tot2);
+      k_label_seg<<<1, 64, 0, S>>>(Kb, tot2, sk, flags + 2);  // empty classes are expected
+      k_label_scatter<<<nb, NT, 0, S>>>(ctx->bkey.as<int>(), N, Kb, ctx->blkcnt2.as<int>(), sk,
+                                        ctx->permk.as<int>());
+      D2MI_HIP(hipGetLastError());
+      hsegk.resize(Kb + 1);
+      D2MI_HIP(hipMemcpyAsync(hsegk.data(), sk, sizeof(int) * (Kb + 1), hipMemcpyDeviceToHost, S));
+      D2MI_HIP(hipStreamSynchronize(S));
+      permk = ctx->permk.as<int>();
+      segk = sk;
+      for (int b = 0; b < nbk; ++b) {
+        bk[b].start = hsegk[b * K];
+        bk[b].count = hsegk[(b + 1) * K] - hsegk[b * K];
+      }
+    } else {
+      bk[0].start = 0;
+      bk[0].count = N;
+    }
+
+    // per bucket: kernel form, column stride, LDS
+    bool any_gm = false;
+    for (int b = 0; b < nbk; ++b) {
+      Bucket &B = bk[b];
+      B.m_max = lims[b];
+      const int sp64 = s + (s & 1) + pad64;
+      B.k64 = s <= 64 && !force_gm && !no_64 &&
+              Plan64Lds(sizeof(T), sp64, B.m_max, s, d, 1, Rmax).total <= ctx->lds_max;
+      const int spl = (s % 2 == 0) ? s + 1 : s;  // odd column stride: conflict-free sums
+      B.gm = !B.k64 &&
+             (force_gm || PlanLds(sizeof(T), spl, B.m_max, s, d, 1, Rmax).total > ctx->lds_max);
+      B.kind = B.gm ? PK_GM : B.k64 ? PK_64 : PK_LEGACY;
+      const int ncols = s <= 64 ? (B.m_max + NT / s - 1) / (NT / s) : 1 << 30;
+      if (B.k64 && !no_reg && PlanRegLds(sizeof(T), sp64, B.m_max, s, d, Rmax).total <= ctx->lds_max) {
+        if (ncols <= 8) B.kind = PK_REG8;
+        else if (!F64 && ncols <= 16) B.kind = PK_REG16;
+      }
+      B.sp = B.k64 ? sp64 : spl;
+      any_gm = any_gm || B.gm;
+    }
+    auto lds_of = [&](const Bucket &B, int keepx, int R) -> size_t {
+      if (B.kind == PK_REG8 || B.kind == PK_REG16)
+        return PlanRegLds(sizeof(T), B.sp, B.m_max, s, d, R).total;
+      return B.k64 ? Plan64Lds(sizeof(T), B.sp, B.m_max, s, d, keepx, R).total
+                   : PlanLds(sizeof(T), B.sp, B.m_max, s, d, keepx, R, B.gm).total;
+    };
+    auto occupancy = [&](const Bucket &B, size_t lds, bool lean, bool resid) {
+      if constexpr (CM == COST_EUCLID) {
+        if (dim_sel == 2) return PlanKernels<T, CM, 2>::occupancy(lds, B.kind, lean, resid);
+        if (dim_sel == 3) return PlanKernels<T, CM, 3>::occupancy(lds, B.kind, lean, resid);
+      }
+      return PlanKernels<T, CM, 0>::occupancy(lds, B.kind, lean, resid);
+    };
+    auto geometry = [&](const Bucket &B, size_t lds, int *G_out, int *chunk_out, bool lean,
+                        bool resid) {
+      const int per_cu = wg_env > 0 ? wg_env : occupancy(B, lds, lean, resid);
+      const int Gt = std::max(1, ctx->num_cu * per_cu);
+      *chunk_out = std::max(1, (B.count + Gt - 1) / Gt);
+      *G_out = (B.count + *chunk_out - 1) / *chunk_out;
+    };
+    if (any_gm) {
+      D2MI_TRY(p->Xout.ensure(sizeof(T) * (size_t)s * p->col));
+      D2MI_TRY(p->Z0.ensure(sizeof(T) * (size_t)s * p->col));
+      xout = p->Xout.as<T>();
+    }
+    // lean passes: FP32 passes after the first with no support update,
+    // residual or kept X (not the HBM-scratch form)
+    long long rec_total = 0;
+    int part_total = 0;
+    for (int b = 0; b < nbk; ++b) {
+      Bucket &B = bk[b];
+      const size_t lds_full = lds_of(B, 1, Rmax);
+      if (lds_full > ctx->lds_max) {
+        d2mi_set_error("B-ADMM with s = %d, d = %d needs %zu B of LDS (limit %zu)", s, d,
+                       lds_full, ctx->lds_max);
+        return D2MI_EUNSUPPORTED;
+      }
+      int chunk_tmp = 0;
+      B.Gmax = 0;
+      if (B.count > 0)
+        geometry(B, lds_of(B, 0, s), &B.Gmax, &chunk_tmp, !F64 && !B.gm && !no_lean, false);
+      B.rec_off = rec_total;
+      rec_total += B.Gmax + K;
+      part_total += B.Gmax;
+    }
+    D2MI_TRY(ctx->rec.ensure(sizeof(double) * (size_t)rec_total * Rmax));
+    D2MI_TRY(ctx->part.ensure(sizeof(double) * (size_t)std::max(setup_blocks, 3 * part_total)));
     evs.resize(2 * (maxIters + 1));
     for (auto &e : evs) D2MI_HIP(hipEventCreate(&e));
     for (int iter = 0; iter <= maxIters; ++iter) {
@@ -2221,36 +2321,49 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
       const int resid = is_resid(iter);
       const int keepx = F64 || upd || resid;
       const int R = upd ? s * (d + 2) : s;
-      const size_t lds = lds_of(keepx, R);
-      const bool lean = lean_ok && iter > 0 && !upd && !resid && !keepx;
-      int G, chunk;
-      geometry(lds, &G, &chunk, lean, resid);
-      if (G > Gmax) {  // records are sized for Gmax workgroups
-        chunk = std::max(1, (N + Gmax - 1) / Gmax);
-        G = (N + chunk - 1) / chunk;
-      }
-      PlanArgs<T> pa{};
-      pa.N = N; pa.s = s; pa.d = d; pa.K = K; pa.sp = sp; pa.m_max = m_max; pa.chunk = chunk;
-      pa.R = R; pa.V = p->V; pa.upd = upd; pa.resid = resid; pa.keepx = keepx;
-      pa.p_str = p->p_str.as<int>(); pa.cum = p->cum.as<long long>(); pa.perm = ctx->perm.as<int>();
-      pa.label = ctx->label.as<int>(); pa.w = w; pa.supp = supp; pa.sym = sym; pa.dist = dist;
-      pa.cw = cwT; pa.csupp = csT; pa.scal = scal; pa.Z = Z; pa.Y = Y;
-      pa.ctab = ctab; pa.xout = xout;
-      pa.xg = gm ? p->Xout.as<T>() : nullptr; pa.z0g = gm ? p->Z0.as<T>() : nullptr;
-      pa.rec = ctx->rec.as<double>(); pa.resid_part = ctx->part.as<double>();
-      pa.first = iter == 0; pa.force_init = fresh ? 1 : 0; pa.lsw = ctx->lsw.as<char>();
+      FinArgs<T> fa{};
+      fa.nbk = nbk;
+      fa.segk = segk;
+      int part_off = 0;
       D2MI_HIP(hipEventRecord(evs[2 * iter], S));
-      if constexpr (CM == COST_EUCLID) {
-        if (dim_sel == 2) PlanKernels<T, CM, 2>::launch(G, lds, S, pa, kind, lean);
-        else if (dim_sel == 3) PlanKernels<T, CM, 3>::launch(G, lds, S, pa, kind, lean);
-        else PlanKernels<T, CM, 0>::launch(G, lds, S, pa, kind, lean);
-      } else {
-        PlanKernels<T, CM, 0>::launch(G, lds, S, pa, kind, lean);
+      for (int b = 0; b < nbk; ++b) {
+        const Bucket &B = bk[b];
+        fa.start[b] = B.start;
+        fa.rec_off[b] = B.rec_off;
+        fa.chunk[b] = 1;
+        if (B.count == 0) continue;
+        const size_t lds = lds_of(B, keepx, R);
+        const bool lean = !F64 && !B.gm && !no_lean && iter > 0 && !upd && !resid && !keepx;
+        int G, chunk;
+        geometry(B, lds, &G, &chunk, lean, resid);
+        if (G > B.Gmax) {  // records are sized for Gmax workgroups
+          chunk = std::max(1, (B.count + B.Gmax - 1) / B.Gmax);
+          G = (B.count + chunk - 1) / chunk;
+        }
+        fa.chunk[b] = chunk;
+        PlanArgs<T> pa{};
+        pa.N = B.count; pa.s = s; pa.d = d; pa.K = K; pa.sp = B.sp; pa.m_max = B.m_max;
+        pa.chunk = chunk; pa.R = R; pa.V = p->V; pa.upd = upd; pa.resid = resid; pa.keepx = keepx;
+        pa.p_str = p->p_str.as<int>(); pa.cum = p->cum.as<long long>(); pa.perm = permk + B.start;
+        pa.label = ctx->label.as<int>(); pa.w = w; pa.supp = supp; pa.sym = sym; pa.dist = dist;
+        pa.cw = cwT; pa.csupp = csT; pa.scal = scal; pa.Z = Z; pa.Y = Y;
+        pa.ctab = ctab; pa.xout = xout;
+        pa.xg = B.gm ? p->Xout.as<T>() : nullptr; pa.z0g = B.gm ? p->Z0.as<T>() : nullptr;
+        pa.rec = ctx->rec.as<double>() + (size_t)B.rec_off * R;
+        pa.resid_part = ctx->part.as<double>() + 3 * part_off;
+        pa.first = iter == 0; pa.force_init = fresh ? 1 : 0; pa.lsw = ctx->lsw.as<char>();
+        if constexpr (CM == COST_EUCLID) {
+          if (dim_sel == 2) PlanKernels<T, CM, 2>::launch(G, lds, S, pa, B.kind, lean);
+          else if (dim_sel == 3) PlanKernels<T, CM, 3>::launch(G, lds, S, pa, B.kind, lean);
+          else PlanKernels<T, CM, 0>::launch(G, lds, S, pa, B.kind, lean);
+        } else {
+          PlanKernels<T, CM, 0>::launch(G, lds, S, pa, B.kind, lean);
+        }
+        part_off += G;
       }
       D2MI_HIP(hipEventRecord(evs[2 * iter + 1], S));
-      FinArgs<T> fa{};
-      fa.K = K; fa.s = s; fa.d = d; fa.chunk = chunk; fa.R = R; fa.G = G; fa.upd = upd;
-      fa.resid = resid; fa.iter = iter; fa.seg = seg; fa.rec = ctx->rec.as<double>();
+      fa.K = K; fa.s = s; fa.d = d; fa.R = R; fa.G = part_off; fa.upd = upd;
+      fa.resid = resid; fa.iter = iter; fa.rec = ctx->rec.as<double>();
       fa.resid_part = ctx->part.as<double>(); fa.cw64 = p->cw64.as<double>(); fa.cwT = cwT;
       fa.cs64 = p->cs64.as<double>(); fa.csT = csT; fa.resid_out = ctx->resid.as<double>();
       fa.rxout = CM == COST_TABLE ? ctx->rxbuf.as<double>() : nullptr;

@@ -79,6 +79,7 @@ struct d2mi_ctx {
   // B-ADMM scratch
   DevMem perm, blkcnt, segstart, rec, part, scal, flags, resid, X32, X64;
   DevMem rec2, rxbuf;           // support-numerator records, X row sums per label
+  DevMem bkey, permk, blkcnt2, segk;  // B-ADMM order by (object-size bucket, label)
   DevMem wtab, wnorm;           // word embeddings: pdist2(vocab, centroid supports), norms
   // pdist2 scratch
   DevMem pA, pB, pC;

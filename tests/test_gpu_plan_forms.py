@@ -1,0 +1,121 @@
+"""The plan kernel's forms on one workload: objects launched by size class
+(register form with 8 or 16 column slots, LDS form, HBM-scratch form) against
+the FP64 oracle, the forms against each other in FP32, and bit-identical
+repeats.  The image-like shape (s = 40, m in [20, 80]) spans two size classes.
+
+Tolerances: FP64 within 1e-9 relative of the oracle; FP32 forms within 1e-5
+of each other on the objective and 1e-4 on supports (their sums run in
+different orders)."""
+import os
+
+import numpy as np
+import pytest
+
+from d2_kmeans_amd import synth
+from d2_kmeans_amd.data import BADMMOptions
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(1e-300, np.max(np.abs(b))))
+
+
+def strokes(n, seed=0):
+    data, meta = synth.grid_strokes(n=n)
+    ph = data.phases[0]
+    c0 = O.centroid_rands(ph, 10, 40, seed=seed)
+    rng = np.random.default_rng(seed)
+    lab = rng.integers(0, 10, size=ph.size).astype(np.int32)
+    return data, ph, c0, lab
+
+
+def update(engine, data, lab, c0, opts, precision, env=None):
+    saved = {}
+    for k, v in (env or {}).items():
+        saved[k] = os.environ.get(k)
+        os.environ[k] = v
+    try:
+        engine.set_data(data)
+        engine.set_centroids(0, c0)
+        engine.set_labels(lab, np.ones(len(lab), np.int8))
+        st = engine.badmm(0, opts, precision=precision)
+        c = engine.get_centroids(0)
+        return st, c, engine.get_plan(0, precision, "Z")
+    finally:
+        for k, v in saved.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+
+
+def test_size_classes_fp64_match_oracle(engine):
+    """FP64, image-like shape: both size classes (register form for m <= 48,
+    LDS form above) against the oracle, including a residual pass."""
+    say("grid strokes N=600 s=40 fp64, size classes vs oracle")
+    data, ph, c0, lab = strokes(600)
+    opts = BADMMOptions(20, 2.0, 10)  # iteration 19 is a residual pass
+    st, cg, _ = update(engine, data, lab, c0, opts, 64)
+    co = c0.copy()
+    ro = O.badmm(ph, lab, np.ones(ph.size, np.int8), co, np.zeros(c0.s * ph.col), opts)
+    assert rel(cg.p_w, co.p_w) < 1e-9
+    assert rel(cg.p_supp, co.p_supp) < 1e-9
+    assert abs(st["obj"] - ro["obj"]) <= 1e-9 * abs(ro["obj"])
+    np.testing.assert_allclose(st["trace"], ro["trace"], rtol=1e-8)
+
+
+@pytest.mark.parametrize("env", [
+    {"D2MI_PLAN_NO_BUCKETS": "1"},                       # one launch, 16-slot register form
+    {"D2MI_PLAN_NOREG": "1"},                            # LDS form
+    {"D2MI_PLAN_LEGACY": "1"},                           # general form
+    {"D2MI_PLAN_LEGACY": "1", "D2MI_BADMM_HBM_SCRATCH": "1"},  # plan scratch in HBM
+], ids=["one_launch", "lds", "general", "hbm_scratch"])
+def test_fp32_forms_agree(engine, env):
+    say(f"grid strokes N=4000 fp32: size classes vs {env}")
+    data, ph, c0, lab = strokes(4000, seed=1)
+    opts = BADMMOptions(10, 2.0, 10)
+    sa, ca, _ = update(engine, data, lab, c0, opts, 32)
+    sb, cb, _ = update(engine, data, lab, c0, opts, 32, env)
+    assert abs(sa["obj"] - sb["obj"]) <= 1e-5 * abs(sb["obj"])
+    assert rel(ca.p_w, cb.p_w) < 1e-5
+    assert rel(ca.p_supp, cb.p_supp) < 1e-4
+
+
+def test_size_classes_bit_reproducible(engine):
+    say("grid strokes N=4000 fp32: repeat")
+    data, ph, c0, lab = strokes(4000, seed=2)
+    opts = BADMMOptions(10, 2.0, 10)
+    outs = [update(engine, data, lab, c0, opts, 32) for _ in range(2)]
+    np.testing.assert_array_equal(outs[0][1].p_w, outs[1][1].p_w)
+    np.testing.assert_array_equal(outs[0][1].p_supp, outs[1][1].p_supp)
+    np.testing.assert_array_equal(outs[0][2], outs[1][2])
+    assert outs[0][0]["obj"] == outs[1][0]["obj"]
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+def test_size_classes_large_objects(engine, precision):
+    """Objects of 40 to 500 points at d = 64 (stored costs), s = 48: four
+    size classes (register, LDS, LDS, HBM scratch) against one launch in the
+    HBM-scratch form, and FP32 against FP64."""
+    say(f"m up to 500, d=64, s=48: size classes, fp{precision}")
+    data, _ = synth.serving_requests(n=160, d=64, m_lo=40, m_hi=500, seed=37)
+    ph = data.phases[0]
+    c0 = synth.seed_centroids(ph, 4, 48)
+    lab, _, _, _ = O.labeling(data, [c0])
+    opts = BADMMOptions(10, 2.0, 10)
+    sa, ca, _ = update(engine, data, lab, c0, opts, precision)
+    sb, cb, _ = update(engine, data, lab, c0, opts, precision, {"D2MI_PLAN_NO_BUCKETS": "1"})
+    tol = 1e-11 if precision == 64 else 1e-5
+    assert abs(sa["obj"] - sb["obj"]) <= tol * abs(sb["obj"])
+    assert rel(ca.p_w, cb.p_w) < tol
+    assert rel(ca.p_supp, cb.p_supp) < (tol if precision == 64 else 1e-4)
+    if precision == 32:
+        s64, c64, _ = update(engine, data, lab, c0, opts, 64)
+        assert abs(sa["obj"] - s64["obj"]) <= 1e-5 * abs(s64["obj"])
+        assert rel(ca.p_supp, c64.p_supp) < 1e-4
