@@ -28,7 +28,7 @@ class BadmmStats(C.Structure):
     _fields_ = [("rho", C.c_double), ("obj", C.c_double), ("n_trace", C.c_int),
                 ("trace", (C.c_double * 4) * 64), ("ms_total", C.c_double),
                 ("ms_plan", C.c_double), ("n_plan_launches", C.c_int),
-                ("plan_elems", C.c_double)]
+                ("plan_elems", C.c_double), ("graph", C.c_int), ("plan_forms", C.c_char * 200)]
 
 
 class LabelStats(C.Structure):
@@ -73,6 +73,9 @@ SIGNATURES = {
     "d2mi_get_centroids": (C.c_int, [_vp, C.c_int, _dp, _dp]),
     "d2mi_badmm": (C.c_int, [_vp, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int,
                              C.POINTER(BadmmStats)]),
+    "d2mi_badmm_set_graph": (C.c_int, [_vp, C.c_int]),
+    "d2mi_badmm_tune": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_char_p, C.c_size_t]),
+    "d2mi_badmm_tune_clear": (C.c_int, [_vp]),
     "d2mi_get_plan": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _dp]),
     "d2mi_set_plan": (C.c_int, [_vp, C.c_int, C.c_int, _dp]),
     "d2mi_pdist2": (C.c_int, [_vp, C.c_int, C.c_size_t, C.c_size_t, _dp, _dp, _dp]),

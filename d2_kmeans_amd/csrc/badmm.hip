@@ -20,7 +20,10 @@
 // ranges do not interleave.  Every sum has a fixed order: results are
 // bit-reproducible run to run.
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
+#include <cstring>
+#include <string>
 #include <type_traits>
 
 #include "d2mi_internal.h"
@@ -1224,15 +1227,32 @@ __global__ __launch_bounds__(NT) void k_badmm_finalize(FinArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem[];
   double *sums = (double *)smem;
   double *red = sums + a.R;
+  double *parts = red + 8;  // [P][R]
   const int l = blockIdx.x, tid = threadIdx.x, s = a.s, d = a.d;
-  for (int q = tid; q < a.R; q += NT) {
+  // P threads per record column: each sums a contiguous share of every
+  // bucket's workgroups (buckets in order), then the shares are added in
+  // order -- a fixed association, and P times fewer dependent round trips to
+  // records written by workgroups on other XCDs
+  const int P = a.R >= NT ? 1 : NT / a.R;
+  for (int t = tid; t < P * a.R; t += NT) {
+    const int q = t % a.R, part = t / a.R;
     double v = 0.0;
-    for (int b = 0; b < a.nbk; ++b) {  // buckets, then workgroups, in order
+    for (int b = 0; b < a.nbk; ++b) {
       const int lo = a.segk[b * a.K + l] - a.start[b], hi = a.segk[b * a.K + l + 1] - a.start[b];
       if (hi <= lo) continue;
-      const int glo = lo / a.chunk[b], ghi = (hi - 1) / a.chunk[b];
-      for (int g = glo; g <= ghi; ++g) v += a.rec[(size_t)(a.rec_off[b] + g + l) * a.R + q];
+      const int glo = lo / a.chunk[b], n = (hi - 1) / a.chunk[b] - glo + 1;
+      const int g0 = glo + (int)((long long)n * part / P);
+      const int g1 = glo + (int)((long long)n * (part + 1) / P);
+      const double *r = a.rec + (size_t)(a.rec_off[b] + l) * a.R + q;  // record (g, l) at g + l
+#pragma unroll 4
+      for (int g = g0; g < g1; ++g) v += r[(size_t)g * a.R];
     }
+    parts[t] = v;
+  }
+  __syncthreads();
+  for (int q = tid; q < a.R; q += NT) {
+    double v = 0.0;
+    for (int p = 0; p < P; ++p) v += parts[p * a.R + q];
     sums[q] = v;
   }
   __syncthreads();
@@ -1675,6 +1695,34 @@ __global__ __launch_bounds__(NT) void k_badmm_setup(SetupArgs<T> a) {
   if (tid == 0) a.part[blockIdx.x] = csum;
 }
 
+// Ground-cost sum of a Euclidean phase (rho, :109) from each object's mean
+// support point mu_i and spread sum_j |x_j - mu_i|^2 (computed at upload):
+// sum_k sum_j |c_k - x_j|^2 = m_i sum_k |c_k - mu_i|^2 + s spread_i, every
+// term non-negative.  One thread per object, one FP64 partial per block.
+__global__ __launch_bounds__(NT) void k_badmm_rho_euclid(int N, int s, int d,
+                                                         const int *__restrict__ p_str,
+                                                         const int *__restrict__ label,
+                                                         const double *__restrict__ mean,
+                                                         const double *__restrict__ spread,
+                                                         const double *__restrict__ cs64,
+                                                         double *__restrict__ part) {
+  __shared__ double red[8];
+  double v = 0.0;
+  for (int o = blockIdx.x * NT + threadIdx.x; o < N; o += gridDim.x * NT) {
+    const double *mu = mean + (size_t)o * d;
+    const double *c = cs64 + (size_t)label[o] * s * d;
+    double a = 0.0;
+    for (int k = 0; k < s; ++k)
+      for (int dd = 0; dd < d; ++dd) {
+        const double u = c[k * d + dd] - mu[dd];
+        a += u * u;
+      }
+    v += (double)p_str[o] * a + (double)s * spread[o];
+  }
+  v = block_sum(v, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = v;
+}
+
 __global__ void k_badmm_rho(const double *__restrict__ part, int nparts, double rhoCoeff,
                             double denom, double *__restrict__ scal, int *flags) {
   __shared__ double red[8];
@@ -1949,6 +1997,22 @@ int occupancy_of(F kernel, size_t lds) {
 // residual or kept X
 enum PlanKind { PK_LEGACY, PK_GM, PK_64, PK_REG8, PK_REG16 };
 
+const char *plan_kind_name(PlanKind k) {
+  switch (k) {
+    case PK_GM: return "hbm";
+    case PK_64: return "lds";
+    case PK_REG8: return "reg8";
+    case PK_REG16: return "reg16";
+    default: return "general";
+  }
+}
+
+// d2mi_badmm_tune's request: passes timed per candidate, and the report
+struct TuneReq {
+  int reps;
+  std::string report;
+};
+
 template <typename T, int CM, int DIM>
 struct PlanKernels {
   // lean instances exist for FP32 only (FP64 passes always keep X), and FP64
@@ -2019,7 +2083,7 @@ struct PlanKernels {
 
 template <typename T, int CM>
 int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int updatePerLoops,
-              int variant, d2mi_badmm_stats *st, bool fresh) {
+              int variant, d2mi_badmm_stats *st, bool fresh, TuneReq *tune) {
   constexpr bool F64 = std::is_same<T, double>::value;
   hipStream_t S = ctx->stream;
   const int N = (int)p->N, K = p->K, s = p->s, d = CM == COST_EUCLID ? p->dim : 0;
@@ -2047,6 +2111,9 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
   // COST_TABLE: rebuild C from the current centroid supports (calculate_distmat)
   const int tab_G = std::max(1, std::min(N, ctx->num_cu * 4));
   const int tab_chunk = (N + tab_G - 1) / tab_G;
+  // support-numerator records, allocated here: nothing allocates inside the
+  // (possibly captured) inner loop
+  if (CM == COST_TABLE) D2MI_TRY(ctx->rec2.ensure(sizeof(double) * (size_t)(tab_G + K) * s * dsup));
   const bool mfma_words = word && !F64 && !getenv_flag("D2MI_WORD_DIRECT");
   if (CM == COST_TABLE && mfma_words) {
     D2MI_TRY(ctx->wtab.ensure(sizeof(float) * (size_t)p->V * K * s));
@@ -2154,8 +2221,19 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
     sa.Z = Z; sa.Y = Y; sa.ctab = ctab; sa.part = ctx->part.as<double>();
     const size_t lds = align16(sizeof(T) * ((size_t)m_max * std::max(d, 1) + (size_t)s * std::max(d, 1)) +
                                sizeof(int) * m_max) + 8 * sizeof(double);
-    k_badmm_setup<T, CM><<<setup_blocks, NT, lds, S>>>(sa);
-    k_badmm_rho<<<1, NT, 0, S>>>(ctx->part.as<double>(), setup_blocks, rhoCoeff,
+    int nparts = setup_blocks;
+    if (variant == D2MI_BADMM_ENGINE && p->metric == D2MI_EUCLIDEAN && p->xmean.p &&
+        !getenv_flag("D2MI_RHO_DIRECT")) {
+      // the engine initialises Z and Y in its first pass: only rho is needed
+      nparts = std::max(1, std::min((N + NT - 1) / NT, ctx->num_cu * 4));
+      k_badmm_rho_euclid<<<nparts, NT, 0, S>>>(N, s, p->dim, p->p_str.as<int>(),
+                                              ctx->label.as<int>(), p->xmean.as<double>(),
+                                              p->xspread.as<double>(), p->cs64.as<double>(),
+                                              ctx->part.as<double>());
+    } else {
+      k_badmm_setup<T, CM><<<setup_blocks, NT, lds, S>>>(sa);
+    }
+    k_badmm_rho<<<1, NT, 0, S>>>(ctx->part.as<double>(), nparts, rhoCoeff,
                                  (double)s * (double)p->col, scal, flags);
     D2MI_HIP(hipGetLastError());
   }
@@ -2164,9 +2242,85 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
     return (iter % 100 == 99) || (iter < 100 && iter % 20 == 19) || iter == maxIters;
   };
 
-  std::vector<hipEvent_t> evs;
   int n_launch = 0;
   double ms_plan = 0.0;
+  bool graphed = false, timed = true;
+  std::string forms;
+
+  // per-pass timing events of the eager loop, kept by the context.  A
+  // captured graph records none: HIP cannot report elapsed times between
+  // events recorded by graph nodes (hipEventElapsedTime answers
+  // hipErrorInvalidHandle, or the times of an earlier eager record of the
+  // same event), so a graphed update reports no per-pass time.
+  while (ctx->pass_ev.size() < 2 * (size_t)(maxIters + 1)) {
+    hipEvent_t e;
+    D2MI_HIP(hipEventCreate(&e));
+    ctx->pass_ev.push_back(e);
+  }
+  hipEvent_t *evs = ctx->pass_ev.data();
+
+  // the inner iterations (:150-320), launched by `loop`; captured into one HIP
+  // graph and replayed where the context allows it (results are the same)
+  auto launch_loop = [&](auto &&loop) -> int {
+    const char *genv = getenv("D2MI_BADMM_GRAPH");
+    const bool want = ctx->badmm_graph >= 0 ? ctx->badmm_graph == 1 : !(genv && atoi(genv) == 0);
+    if (want && !tune) {
+      hipGraph_t g = nullptr;
+      const auto key = std::make_tuple((int)(p - ctx->ph.data()), (int)sizeof(T) * 8 + 100 * CM,
+                                       maxIters, updatePerLoops);
+      auto &entry = ctx->badmm_exec[key];
+      const bool dbg = getenv_flag("D2MI_DEBUG_GRAPH");
+      const hipError_t eb = hipStreamBeginCapture(S, hipStreamCaptureModeThreadLocal);
+      if (dbg) fprintf(stderr, "[d2mi graph] begin capture: %s\n", hipGetErrorName(eb));
+      if (eb == hipSuccess) {
+        timed = false;  // no event nodes in the graph
+        const int rc = loop();
+        const hipError_t ec = hipStreamEndCapture(S, &g);
+        if (dbg)
+          fprintf(stderr, "[d2mi graph] loop rc %d (%s), end capture: %s\n", rc,
+                  rc ? d2mi_last_error() : "", hipGetErrorName(ec));
+        if (rc == D2MI_OK && ec == hipSuccess && g) {
+          hipGraphExec_t &ex = entry.exec;
+          bool ok = false;
+          if (ex) {
+            hipGraphNode_t en = nullptr;
+            hipGraphExecUpdateResult ur;
+            const hipError_t eu = hipGraphExecUpdate(ex, g, &en, &ur);
+            ok = eu == hipSuccess && ur == hipGraphExecUpdateSuccess;
+            if (dbg) fprintf(stderr, "[d2mi graph] exec update: %s result %d\n", hipGetErrorName(eu), (int)ur);
+            if (!ok) {
+              (void)hipGetLastError();
+              (void)hipGraphExecDestroy(ex);
+              ex = nullptr;
+            }
+          }
+          if (!ok) {
+            const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+            ok = ei == hipSuccess;
+            if (dbg) fprintf(stderr, "[d2mi graph] instantiate: %s\n", hipGetErrorName(ei));
+            if (!ok) {
+              (void)hipGetLastError();
+              ex = nullptr;
+            }
+          }
+          (void)hipGraphDestroy(g);
+          if (ok) {
+            D2MI_HIP(hipGraphLaunch(ex, S));
+            graphed = true;
+            return D2MI_OK;
+          }
+        } else if (g) {
+          (void)hipGraphDestroy(g);
+        }
+        (void)hipGetLastError();  // capture refused: launch one by one
+        timed = true;
+      } else {
+        (void)hipGetLastError();
+      }
+      n_launch = 0;
+    }
+    return loop();
+  };
 
   if (variant == D2MI_BADMM_ENGINE) {
     const int Rmax = euclid_upd ? s * (d + 2) : s;
@@ -2188,6 +2342,7 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
     const bool force_gm = getenv_flag("D2MI_BADMM_HBM_SCRATCH");
     const bool no_64 = getenv_flag("D2MI_PLAN_LEGACY"), no_reg = getenv_flag("D2MI_PLAN_NOREG");
     const bool no_lean = getenv_flag("D2MI_PLAN_NO_LEAN");
+    const bool forced = force_gm || no_64 || no_reg;  // tuned choices give way to these
     // fixed Euclidean dimensions get straight-line cost code
     const int dim_sel = CM == COST_EUCLID && (d == 2 || d == 3) ? d : 0;
 
@@ -2206,7 +2361,7 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
     lims.push_back(m_max);
     const int nbk = (int)lims.size();
     struct Bucket {
-      int m_max, start, count, sp, Gmax;
+      int m_max, start, count, sp, Gmax, wg_mode;
       bool k64, gm;
       PlanKind kind;
       long long rec_off;
@@ -2247,24 +2402,60 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
       bk[0].count = N;
     }
 
-    // per bucket: kernel form, column stride, LDS
+    // ---- kernel forms: which fit a class (LDS, columns per thread), the
+    // rule-based choice, and the tuner's choice where it has one
+    const int sp64 = s + (s & 1) + pad64;
+    const int spl = (s % 2 == 0) ? s + 1 : s;  // odd column stride: conflict-free sums
+    auto set_kind = [&](Bucket &B, PlanKind k) {
+      B.kind = k;
+      B.k64 = k == PK_64 || k == PK_REG8 || k == PK_REG16;
+      B.gm = k == PK_GM;
+      B.sp = B.k64 ? sp64 : spl;
+    };
+    auto fits = [&](int m_lim, PlanKind k) -> bool {
+      const int ncols = s <= 64 ? (m_lim + NT / s - 1) / (NT / s) : 1 << 30;
+      switch (k) {
+        case PK_REG16:
+          if (F64) return false;
+          [[fallthrough]];
+        case PK_REG8:
+          return s <= 64 && ncols <= (k == PK_REG8 ? 8 : 16) &&
+                 Plan64Lds(sizeof(T), sp64, m_lim, s, d, 1, Rmax).total <= ctx->lds_max &&
+                 PlanRegLds(sizeof(T), sp64, m_lim, s, d, Rmax).total <= ctx->lds_max;
+        case PK_64:
+          return s <= 64 && Plan64Lds(sizeof(T), sp64, m_lim, s, d, 1, Rmax).total <= ctx->lds_max;
+        case PK_LEGACY:
+          return PlanLds(sizeof(T), spl, m_lim, s, d, 1, Rmax).total <= ctx->lds_max;
+        default:
+          return PlanLds(sizeof(T), spl, m_lim, s, d, 1, Rmax, true).total <= ctx->lds_max;
+      }
+    };
+    auto tune_key = [&](int m_lim) {
+      return PlanTuneKey{(int)sizeof(T) * 8, CM, dim_sel, s, d, m_lim};
+    };
     bool any_gm = false;
     for (int b = 0; b < nbk; ++b) {
       Bucket &B = bk[b];
       B.m_max = lims[b];
-      const int sp64 = s + (s & 1) + pad64;
-      B.k64 = s <= 64 && !force_gm && !no_64 &&
-              Plan64Lds(sizeof(T), sp64, B.m_max, s, d, 1, Rmax).total <= ctx->lds_max;
-      const int spl = (s % 2 == 0) ? s + 1 : s;  // odd column stride: conflict-free sums
-      B.gm = !B.k64 &&
-             (force_gm || PlanLds(sizeof(T), spl, B.m_max, s, d, 1, Rmax).total > ctx->lds_max);
-      B.kind = B.gm ? PK_GM : B.k64 ? PK_64 : PK_LEGACY;
-      const int ncols = s <= 64 ? (B.m_max + NT / s - 1) / (NT / s) : 1 << 30;
-      if (B.k64 && !no_reg && PlanRegLds(sizeof(T), sp64, B.m_max, s, d, Rmax).total <= ctx->lds_max) {
-        if (ncols <= 8) B.kind = PK_REG8;
-        else if (!F64 && ncols <= 16) B.kind = PK_REG16;
+      B.wg_mode = 0;
+      PlanKind k = PK_GM;
+      if (!force_gm) {
+        if (!no_64 && fits(B.m_max, PK_64)) {
+          k = PK_64;
+          if (!no_reg && fits(B.m_max, PK_REG8)) k = PK_REG8;
+          else if (!no_reg && fits(B.m_max, PK_REG16)) k = PK_REG16;
+        } else if (fits(B.m_max, PK_LEGACY)) {
+          k = PK_LEGACY;
+        }
       }
-      B.sp = B.k64 ? sp64 : spl;
+      if (!forced && !tune) {
+        auto it = ctx->plan_tune.find(tune_key(B.m_max));
+        if (it != ctx->plan_tune.end() && fits(B.m_max, (PlanKind)it->second.kind)) {
+          k = (PlanKind)it->second.kind;
+          B.wg_mode = it->second.wg_mode;
+        }
+      }
+      set_kind(B, k);
       any_gm = any_gm || B.gm;
     }
     auto lds_of = [&](const Bucket &B, int keepx, int R) -> size_t {
@@ -2280,18 +2471,126 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
       }
       return PlanKernels<T, CM, 0>::occupancy(lds, B.kind, lean, resid);
     };
+    auto per_cu_of = [&](const Bucket &B, size_t lds, bool lean, bool resid) {
+      if (wg_env > 0) return wg_env;
+      const int occ = occupancy(B, lds, lean, resid);
+      return B.wg_mode == 1 ? std::max(1, occ - 1) : B.wg_mode == 2 ? 2 * occ : occ;
+    };
     auto geometry = [&](const Bucket &B, size_t lds, int *G_out, int *chunk_out, bool lean,
                         bool resid) {
-      const int per_cu = wg_env > 0 ? wg_env : occupancy(B, lds, lean, resid);
-      const int Gt = std::max(1, ctx->num_cu * per_cu);
+      const int Gt = std::max(1, ctx->num_cu * per_cu_of(B, lds, lean, resid));
       *chunk_out = std::max(1, (B.count + Gt - 1) / Gt);
       *G_out = (B.count + *chunk_out - 1) / *chunk_out;
+    };
+    auto launch_plan = [&](int G, size_t lds, const PlanArgs<T> &pa, PlanKind kind, bool lean) {
+      if constexpr (CM == COST_EUCLID) {
+        if (dim_sel == 2) return PlanKernels<T, CM, 2>::launch(G, lds, S, pa, kind, lean);
+        if (dim_sel == 3) return PlanKernels<T, CM, 3>::launch(G, lds, S, pa, kind, lean);
+      }
+      PlanKernels<T, CM, 0>::launch(G, lds, S, pa, kind, lean);
+    };
+    // the plan-kernel arguments of bucket B for one pass
+    auto plan_args = [&](const Bucket &B, int chunk, int R, int upd, int resid, int keepx,
+                         int first) {
+      PlanArgs<T> pa{};
+      pa.N = B.count; pa.s = s; pa.d = d; pa.K = K; pa.sp = B.sp; pa.m_max = B.m_max;
+      pa.chunk = chunk; pa.R = R; pa.V = p->V; pa.upd = upd; pa.resid = resid; pa.keepx = keepx;
+      pa.p_str = p->p_str.as<int>(); pa.cum = p->cum.as<long long>(); pa.perm = permk + B.start;
+      pa.label = ctx->label.as<int>(); pa.w = w; pa.supp = supp; pa.sym = sym; pa.dist = dist;
+      pa.cw = cwT; pa.csupp = csT; pa.scal = scal; pa.Z = Z; pa.Y = Y;
+      pa.ctab = ctab; pa.xout = xout;
+      pa.xg = B.gm ? p->Xout.as<T>() : nullptr; pa.z0g = B.gm ? p->Z0.as<T>() : nullptr;
+      pa.rec = ctx->rec.as<double>() + (size_t)B.rec_off * R;
+      pa.first = first; pa.force_init = fresh ? 1 : 0; pa.lsw = ctx->lsw.as<char>();
+      return pa;
     };
     if (any_gm) {
       D2MI_TRY(p->Xout.ensure(sizeof(T) * (size_t)s * p->col));
       D2MI_TRY(p->Z0.ensure(sizeof(T) * (size_t)s * p->col));
       xout = p->Xout.as<T>();
     }
+
+    // ---- tuner: time every feasible (form, workgroups per CU) of each class
+    // on lean passes over the resident plans, then put the plans back
+    if (tune) {
+      if (fresh) {
+        d2mi_set_error("d2mi_badmm_tune: run one update of this phase at this precision first");
+        return D2MI_EINVAL;
+      }
+      const size_t bytes = sizeof(T) * (size_t)s * p->col;
+      D2MI_TRY(ctx->tuneZ.ensure(bytes));
+      D2MI_TRY(ctx->tuneY.ensure(bytes));
+      D2MI_HIP(hipMemcpyAsync(ctx->tuneZ.p, Z, bytes, hipMemcpyDeviceToDevice, S));
+      D2MI_HIP(hipMemcpyAsync(ctx->tuneY.p, Y, bytes, hipMemcpyDeviceToDevice, S));
+      const int reps = std::max(1, tune->reps);
+      char line[256];
+      for (int b = 0; b < nbk; ++b) {
+        Bucket B = bk[b];
+        if (B.count == 0) continue;
+        snprintf(line, sizeof line, "m<=%d (%d objects):", B.m_max, B.count);
+        tune->report += line;
+        // candidates: the rule-based form first (the incumbent), then every
+        // other feasible (form, workgroups per CU); a challenger must be more
+        // than 3% faster on the plan passes, since more workgroups also cost
+        // the per-pass reduction (more records) that these timings leave out
+        std::vector<PlanTuneVal> cands{PlanTuneVal{(int)B.kind, 0}};
+        const PlanKind kinds[] = {PK_REG8, PK_REG16, PK_64, PK_LEGACY, PK_GM};
+        bool any_fast = false;
+        for (PlanKind k : kinds)
+          if (k != PK_GM && fits(B.m_max, k)) any_fast = true;
+        for (PlanKind k : kinds) {
+          if (!fits(B.m_max, k) || (k == PK_GM && any_fast)) continue;
+          for (int wm = 0; wm < 3; ++wm)
+            if (!(k == B.kind && wm == 0)) cands.push_back(PlanTuneVal{(int)k, wm});
+        }
+        float best = 0.f;
+        PlanTuneVal bestv = cands[0];
+        for (size_t ci = 0; ci < cands.size(); ++ci) {
+          const PlanKind k = (PlanKind)cands[ci].kind;
+          const int wm = cands[ci].wg_mode;
+          if (k == PK_GM) {  // the HBM-scratch form keeps x and Z0 in plan-sized arrays
+            D2MI_TRY(p->Xout.ensure(sizeof(T) * (size_t)s * p->col));
+            D2MI_TRY(p->Z0.ensure(sizeof(T) * (size_t)s * p->col));
+            xout = p->Xout.as<T>();
+          }
+          set_kind(B, k);
+          B.wg_mode = wm;
+          B.rec_off = 0;
+          const bool lean = !F64 && !B.gm && !no_lean;
+          const size_t lds = lds_of(B, F64 ? 1 : 0, s);
+          if (wm == 1 && occupancy(B, lds, lean, false) <= 1) continue;
+          int G, chunk;
+          geometry(B, lds, &G, &chunk, lean, false);
+          D2MI_TRY(ctx->rec.ensure(sizeof(double) * (size_t)(G + K) * s));
+          const PlanArgs<T> pa = plan_args(B, chunk, s, 0, 0, F64 ? 1 : 0, 0);
+          launch_plan(G, lds, pa, k, lean);  // warm
+          D2MI_HIP(hipEventRecord(ctx->ev[2], S));
+          for (int r = 0; r < reps; ++r) launch_plan(G, lds, pa, k, lean);
+          D2MI_HIP(hipEventRecord(ctx->ev[3], S));
+          D2MI_HIP(hipGetLastError());
+          D2MI_HIP(hipEventSynchronize(ctx->ev[3]));
+          float ms = 0.f;
+          D2MI_HIP(hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]));
+          ms /= reps;
+          snprintf(line, sizeof line, " %s/%s %.4f", plan_kind_name(k),
+                   wm == 0 ? "occ" : wm == 1 ? "occ-1" : "2occ", ms);
+          tune->report += line;
+          if (ci == 0 || ms < 0.97f * best) {
+            best = ms;
+            bestv = cands[ci];
+          }
+        }
+        ctx->plan_tune[tune_key(B.m_max)] = bestv;
+        snprintf(line, sizeof line, " -> %s/%s\n", plan_kind_name((PlanKind)bestv.kind),
+                 bestv.wg_mode == 0 ? "occ" : bestv.wg_mode == 1 ? "occ-1" : "2occ");
+        tune->report += line;
+      }
+      D2MI_HIP(hipMemcpyAsync(Z, ctx->tuneZ.p, bytes, hipMemcpyDeviceToDevice, S));
+      D2MI_HIP(hipMemcpyAsync(Y, ctx->tuneY.p, bytes, hipMemcpyDeviceToDevice, S));
+      D2MI_HIP(hipStreamSynchronize(S));
+      return D2MI_OK;
+    }
+
     // lean passes: FP32 passes after the first with no support update,
     // residual or kept X (not the HBM-scratch form)
     long long rec_total = 0;
@@ -2311,71 +2610,66 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
       B.rec_off = rec_total;
       rec_total += B.Gmax + K;
       part_total += B.Gmax;
+      if (B.count > 0) {
+        char f[64];
+        snprintf(f, sizeof f, "%sm<=%d:%s/%d(%d)", forms.empty() ? "" : " ", B.m_max,
+                 plan_kind_name(B.kind), per_cu_of(B, lds_of(B, 0, s), !F64 && !B.gm && !no_lean, false),
+                 B.count);
+        forms += f;
+      }
     }
     D2MI_TRY(ctx->rec.ensure(sizeof(double) * (size_t)rec_total * Rmax));
     D2MI_TRY(ctx->part.ensure(sizeof(double) * (size_t)std::max(setup_blocks, 3 * part_total)));
-    evs.resize(2 * (maxIters + 1));
-    for (auto &e : evs) D2MI_HIP(hipEventCreate(&e));
-    for (int iter = 0; iter <= maxIters; ++iter) {
-      const int upd = euclid_upd && (iter % updatePerLoops == 0);
-      const int resid = is_resid(iter);
-      const int keepx = F64 || upd || resid;
-      const int R = upd ? s * (d + 2) : s;
-      FinArgs<T> fa{};
-      fa.nbk = nbk;
-      fa.segk = segk;
-      int part_off = 0;
-      D2MI_HIP(hipEventRecord(evs[2 * iter], S));
-      for (int b = 0; b < nbk; ++b) {
-        const Bucket &B = bk[b];
-        fa.start[b] = B.start;
-        fa.rec_off[b] = B.rec_off;
-        fa.chunk[b] = 1;
-        if (B.count == 0) continue;
-        const size_t lds = lds_of(B, keepx, R);
-        const bool lean = !F64 && !B.gm && !no_lean && iter > 0 && !upd && !resid && !keepx;
-        int G, chunk;
-        geometry(B, lds, &G, &chunk, lean, resid);
-        if (G > B.Gmax) {  // records are sized for Gmax workgroups
-          chunk = std::max(1, (B.count + B.Gmax - 1) / B.Gmax);
-          G = (B.count + chunk - 1) / chunk;
+    auto loop = [&]() -> int {
+      n_launch = 0;
+      for (int iter = 0; iter <= maxIters; ++iter) {
+        const int upd = euclid_upd && (iter % updatePerLoops == 0);
+        const int resid = is_resid(iter);
+        const int keepx = F64 || upd || resid;
+        const int R = upd ? s * (d + 2) : s;
+        FinArgs<T> fa{};
+        fa.nbk = nbk;
+        fa.segk = segk;
+        int part_off = 0;
+        if (timed) D2MI_HIP(hipEventRecord(evs[2 * iter], S));
+        for (int b = 0; b < nbk; ++b) {
+          const Bucket &B = bk[b];
+          fa.start[b] = B.start;
+          fa.rec_off[b] = B.rec_off;
+          fa.chunk[b] = 1;
+          if (B.count == 0) continue;
+          const size_t lds = lds_of(B, keepx, R);
+          const bool lean = !F64 && !B.gm && !no_lean && iter > 0 && !upd && !resid && !keepx;
+          int G, chunk;
+          geometry(B, lds, &G, &chunk, lean, resid);
+          if (G > B.Gmax) {  // records are sized for Gmax workgroups
+            chunk = std::max(1, (B.count + B.Gmax - 1) / B.Gmax);
+            G = (B.count + chunk - 1) / chunk;
+          }
+          fa.chunk[b] = chunk;
+          PlanArgs<T> pa = plan_args(B, chunk, R, upd, resid, keepx, iter == 0);
+          pa.resid_part = ctx->part.as<double>() + 3 * part_off;
+          launch_plan(G, lds, pa, B.kind, lean);
+          part_off += G;
         }
-        fa.chunk[b] = chunk;
-        PlanArgs<T> pa{};
-        pa.N = B.count; pa.s = s; pa.d = d; pa.K = K; pa.sp = B.sp; pa.m_max = B.m_max;
-        pa.chunk = chunk; pa.R = R; pa.V = p->V; pa.upd = upd; pa.resid = resid; pa.keepx = keepx;
-        pa.p_str = p->p_str.as<int>(); pa.cum = p->cum.as<long long>(); pa.perm = permk + B.start;
-        pa.label = ctx->label.as<int>(); pa.w = w; pa.supp = supp; pa.sym = sym; pa.dist = dist;
-        pa.cw = cwT; pa.csupp = csT; pa.scal = scal; pa.Z = Z; pa.Y = Y;
-        pa.ctab = ctab; pa.xout = xout;
-        pa.xg = B.gm ? p->Xout.as<T>() : nullptr; pa.z0g = B.gm ? p->Z0.as<T>() : nullptr;
-        pa.rec = ctx->rec.as<double>() + (size_t)B.rec_off * R;
-        pa.resid_part = ctx->part.as<double>() + 3 * part_off;
-        pa.first = iter == 0; pa.force_init = fresh ? 1 : 0; pa.lsw = ctx->lsw.as<char>();
-        if constexpr (CM == COST_EUCLID) {
-          if (dim_sel == 2) PlanKernels<T, CM, 2>::launch(G, lds, S, pa, B.kind, lean);
-          else if (dim_sel == 3) PlanKernels<T, CM, 3>::launch(G, lds, S, pa, B.kind, lean);
-          else PlanKernels<T, CM, 0>::launch(G, lds, S, pa, B.kind, lean);
-        } else {
-          PlanKernels<T, CM, 0>::launch(G, lds, S, pa, B.kind, lean);
+        if (timed) D2MI_HIP(hipEventRecord(evs[2 * iter + 1], S));
+        fa.K = K; fa.s = s; fa.d = d; fa.R = R; fa.G = part_off; fa.upd = upd;
+        fa.resid = resid; fa.iter = iter; fa.rec = ctx->rec.as<double>();
+        fa.resid_part = ctx->part.as<double>(); fa.cw64 = p->cw64.as<double>(); fa.cwT = cwT;
+        fa.cs64 = p->cs64.as<double>(); fa.csT = csT; fa.resid_out = ctx->resid.as<double>();
+        fa.rxout = CM == COST_TABLE ? ctx->rxbuf.as<double>() : nullptr;
+        fa.flags = flags;
+        k_badmm_finalize<T><<<K, NT, sizeof(double) * (R + 8 + std::max(NT, R)), S>>>(fa);
+        D2MI_HIP(hipGetLastError());
+        if (CM == COST_TABLE && upd) {  // step 5 (:196-227): supports, then C again
+          D2MI_TRY(update_supports(S, seg, flags));
+          D2MI_TRY(build_table(S));
         }
-        part_off += G;
+        ++n_launch;
       }
-      D2MI_HIP(hipEventRecord(evs[2 * iter + 1], S));
-      fa.K = K; fa.s = s; fa.d = d; fa.R = R; fa.G = part_off; fa.upd = upd;
-      fa.resid = resid; fa.iter = iter; fa.rec = ctx->rec.as<double>();
-      fa.resid_part = ctx->part.as<double>(); fa.cw64 = p->cw64.as<double>(); fa.cwT = cwT;
-      fa.cs64 = p->cs64.as<double>(); fa.csT = csT; fa.resid_out = ctx->resid.as<double>();
-      fa.rxout = CM == COST_TABLE ? ctx->rxbuf.as<double>() : nullptr;
-      fa.flags = flags;
-      k_badmm_finalize<T><<<K, NT, sizeof(double) * (R + 8), S>>>(fa);
-      D2MI_HIP(hipGetLastError());
-      if (CM == COST_TABLE && upd) {  // step 5 (:196-227): supports, then C again
-        D2MI_TRY(update_supports(S, seg, flags));
-        D2MI_TRY(build_table(S));
-      }
-      ++n_launch;
-    }
+      return D2MI_OK;
+    };
+    D2MI_TRY(launch_loop(loop));
   } else {
     // ---- plain form
     const size_t n = (size_t)s * p->col;
@@ -2384,8 +2678,7 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
     const size_t accn = (size_t)K * s * (dsup + 2);
     D2MI_TRY(ctx->rec.ensure(sizeof(double) * accn));
     k_zero_f64<<<256, NT, 0, S>>>(ctx->rec.as<double>(), accn);
-    evs.resize(2 * (maxIters + 1));
-    for (auto &e : evs) D2MI_HIP(hipEventCreate(&e));
+    forms = "plain";
     for (int iter = 0; iter <= maxIters; ++iter) {
       const int upd = euclid_upd && (iter % updatePerLoops == 0);
       PlainArgs<T> pa{};
@@ -2407,12 +2700,17 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
   }
   D2MI_HIP(hipEventRecord(ctx->ev[1], S));
   D2MI_HIP(hipStreamSynchronize(S));
-  for (int i = 0; i < n_launch; ++i) {
+  for (int i = 0; i < n_launch && timed; ++i) {
     float ms = 0.f;
-    D2MI_HIP(hipEventElapsedTime(&ms, evs[2 * i], evs[2 * i + 1]));
+    if (hipEventElapsedTime(&ms, evs[2 * i], evs[2 * i + 1]) != hipSuccess) {
+      // a graph that cannot report its recorded events' times: no per-pass time
+      (void)hipGetLastError();
+      timed = false;
+      break;
+    }
     ms_plan += ms;
   }
-  for (auto &e : evs) (void)hipEventDestroy(e);
+  if (!timed) ms_plan = -1.0;
   D2MI_HIP(hipMemcpy(&hflags, flags, sizeof(int), hipMemcpyDeviceToHost));
   if (hflags & FLAG_NUMERIC) {
     d2mi_set_error("a normaliser was not positive (blas_like64.c:131/148 assert)");
@@ -2427,6 +2725,8 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
     st->ms_total = ms;
     st->ms_plan = ms_plan;
     st->n_plan_launches = n_launch;
+    st->graph = graphed ? 1 : 0;
+    snprintf(st->plan_forms, sizeof st->plan_forms, "%s", forms.c_str());
     st->plan_elems = (double)s * (double)p->col;
     st->n_trace = 0;
     st->obj = 0.0;
@@ -2453,9 +2753,9 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
 
 }  // namespace
 
-extern "C" int d2mi_badmm(d2mi_ctx *ctx, int phase, int maxIters, double rhoCoeff,
-                          int updatePerLoops, int precision, int variant,
-                          d2mi_badmm_stats *stats) {
+namespace {
+int badmm_dispatch(d2mi_ctx *ctx, int phase, int maxIters, double rhoCoeff, int updatePerLoops,
+                   int precision, int variant, d2mi_badmm_stats *stats, TuneReq *tune) {
   D2MI_TRY(d2mi_check_ctx(ctx));
   PhaseDev *p = d2mi_phase(ctx, phase);
   if (!p) return D2MI_EINVAL;
@@ -2488,30 +2788,66 @@ extern "C" int d2mi_badmm(d2mi_ctx *ctx, int phase, int maxIters, double rhoCoef
     d2mi_set_error("K = %d too large", p->K);
     return D2MI_EUNSUPPORTED;
   }
+  if (tune && (precision == 64 ? p->plan_s64 : p->plan_s32) != p->s) {
+    d2mi_set_error("d2mi_badmm_tune: run one update of this phase at this precision first");
+    return D2MI_EINVAL;
+  }
   const int fresh = d2mi_ensure_plans(ctx, p, precision);
   if (fresh < 0) return fresh;
   const bool f = fresh == 1;
   if (table) {
     if (precision == 64)
-      return run_badmm<double, COST_TABLE>(ctx, p, maxIters, rhoCoeff, updatePerLoops, variant, stats, f);
-    return run_badmm<float, COST_TABLE>(ctx, p, maxIters, rhoCoeff, updatePerLoops, variant, stats, f);
+      return run_badmm<double, COST_TABLE>(ctx, p, maxIters, rhoCoeff, updatePerLoops, variant, stats, f, tune);
+    return run_badmm<float, COST_TABLE>(ctx, p, maxIters, rhoCoeff, updatePerLoops, variant, stats, f, tune);
   }
   if (precision == 64) {
     switch (p->metric) {
       case D2MI_EUCLIDEAN:
-        return run_badmm<double, COST_EUCLID>(ctx, p, maxIters, rhoCoeff, updatePerLoops, variant, stats, f);
+        return run_badmm<double, COST_EUCLID>(ctx, p, maxIters, rhoCoeff, updatePerLoops, variant, stats, f, tune);
       case D2MI_HISTOGRAM:
-        return run_badmm<double, COST_HIST>(ctx, p, maxIters, rhoCoeff, updatePerLoops, variant, stats, f);
+        return run_badmm<double, COST_HIST>(ctx, p, maxIters, rhoCoeff, updatePerLoops, variant, stats, f, tune);
       default:
-        return run_badmm<double, COST_SPARSE>(ctx, p, maxIters, rhoCoeff, updatePerLoops, variant, stats, f);
+        return run_badmm<double, COST_SPARSE>(ctx, p, maxIters, rhoCoeff, updatePerLoops, variant, stats, f, tune);
     }
   }
   switch (p->metric) {
     case D2MI_EUCLIDEAN:
-      return run_badmm<float, COST_EUCLID>(ctx, p, maxIters, rhoCoeff, updatePerLoops, variant, stats, f);
+      return run_badmm<float, COST_EUCLID>(ctx, p, maxIters, rhoCoeff, updatePerLoops, variant, stats, f, tune);
     case D2MI_HISTOGRAM:
-      return run_badmm<float, COST_HIST>(ctx, p, maxIters, rhoCoeff, updatePerLoops, variant, stats, f);
+      return run_badmm<float, COST_HIST>(ctx, p, maxIters, rhoCoeff, updatePerLoops, variant, stats, f, tune);
     default:
-      return run_badmm<float, COST_SPARSE>(ctx, p, maxIters, rhoCoeff, updatePerLoops, variant, stats, f);
+      return run_badmm<float, COST_SPARSE>(ctx, p, maxIters, rhoCoeff, updatePerLoops, variant, stats, f, tune);
   }
+}
+}  // namespace
+
+extern "C" int d2mi_badmm(d2mi_ctx *ctx, int phase, int maxIters, double rhoCoeff,
+                          int updatePerLoops, int precision, int variant,
+                          d2mi_badmm_stats *stats) {
+  if (stats) memset(stats, 0, sizeof *stats);
+  return badmm_dispatch(ctx, phase, maxIters, rhoCoeff, updatePerLoops, precision, variant,
+                        stats, nullptr);
+}
+
+extern "C" int d2mi_badmm_tune(d2mi_ctx *ctx, int phase, int precision, int reps, char *report,
+                               size_t report_len) {
+  TuneReq tq{reps, std::string()};
+  // the setup of an update (label order, rho, stored costs) with the
+  // reference's default options, then the timing passes; no iteration runs
+  const int rc = badmm_dispatch(ctx, phase, 10, 2.0, 10, precision, D2MI_BADMM_ENGINE, nullptr,
+                                &tq);
+  if (report && report_len) snprintf(report, report_len, "%s", tq.report.c_str());
+  return rc;
+}
+
+extern "C" int d2mi_badmm_tune_clear(d2mi_ctx *ctx) {
+  D2MI_TRY(d2mi_check_ctx(ctx));
+  ctx->plan_tune.clear();
+  return D2MI_OK;
+}
+
+extern "C" int d2mi_badmm_set_graph(d2mi_ctx *ctx, int on) {
+  D2MI_TRY(d2mi_check_ctx(ctx));
+  ctx->badmm_graph = on ? 1 : 0;
+  return D2MI_OK;
 }

@@ -96,7 +96,8 @@ extern "C" int d2mi_create(d2mi_ctx **out, int device) {
 static void release_phase(PhaseDev &p) {
   DevMem *all[] = {&p.p_str, &p.cum, &p.w64, &p.w32, &p.supp64, &p.supp32, &p.sym,
                    &p.vocab64, &p.vocab32, &p.dist64, &p.dist32, &p.cw64, &p.cw32,
-                   &p.cs64, &p.cs32, &p.Z32, &p.Y32, &p.Z64, &p.Y64, &p.cw_old, &p.cs_old};
+                   &p.cs64, &p.cs32, &p.Z32, &p.Y32, &p.Z64, &p.Y64, &p.cw_old, &p.cs_old,
+                   &p.Ctab, &p.Xout, &p.Z0, &p.bucket_map, &p.xmean, &p.xspread};
   for (DevMem *m : all) m->release();
   p = PhaseDev();
 }
@@ -110,10 +111,15 @@ extern "C" int d2mi_destroy(d2mi_ctx *ctx) {
                    &ctx->rec, &ctx->part, &ctx->scal, &ctx->flags, &ctx->resid,
                    &ctx->X32, &ctx->X64, &ctx->pA, &ctx->pB, &ctx->pC, &ctx->el_l,
                    &ctx->el_u, &ctx->el_s, &ctx->el_c, &ctx->el_r, &ctx->dist, &ctx->mask,
-                   &ctx->jobs, &ctx->lpstat, &ctx->cnt, &ctx->mind2, &ctx->red};
+                   &ctx->jobs, &ctx->lpstat, &ctx->cnt, &ctx->mind2, &ctx->red,
+                   &ctx->rec2, &ctx->rxbuf, &ctx->bkey, &ctx->permk, &ctx->blkcnt2,
+                   &ctx->segk, &ctx->wtab, &ctx->wnorm, &ctx->tuneZ, &ctx->tuneY};
   for (DevMem *m : all) m->release();
   for (auto &e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
+  for (auto &kv : ctx->badmm_exec)
+    if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+  for (auto &e : ctx->pass_ev) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return D2MI_OK;
@@ -130,6 +136,35 @@ __global__ void k_to_f32(const double *__restrict__ a, float *__restrict__ b, si
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x)
     b[i] = (float)a[i];
+}
+
+// Per object of a Euclidean phase: the mean support point mu (unweighted)
+// and sum_j |x_j - mu|^2, in FP64.  They give the ground-cost sum of an
+// object against any s centroid points without forming its s x m costs:
+// sum_k sum_j |c_k - x_j|^2 = m sum_k |c_k - mu|^2 + s sum_j |x_j - mu|^2
+// (the cross term vanishes because sum_j (x_j - mu) = 0), a sum of
+// non-negative terms with no cancellation.
+__global__ void k_obj_moments(size_t N, int d, const int *__restrict__ p_str,
+                              const long long *__restrict__ cum, const double *__restrict__ supp,
+                              double *__restrict__ mean, double *__restrict__ spread) {
+  for (size_t o = blockIdx.x * (size_t)blockDim.x + threadIdx.x; o < N;
+       o += (size_t)gridDim.x * blockDim.x) {
+    const int m = p_str[o];
+    const double *x = supp + (size_t)cum[o] * d;
+    double *mu = mean + o * d;
+    for (int dd = 0; dd < d; ++dd) {
+      double a = 0.0;
+      for (int j = 0; j < m; ++j) a += x[(size_t)j * d + dd];
+      mu[dd] = a / m;
+    }
+    double v = 0.0;
+    for (int j = 0; j < m; ++j)
+      for (int dd = 0; dd < d; ++dd) {
+        const double u = x[(size_t)j * d + dd] - mu[dd];
+        v += u * u;
+      }
+    spread[o] = v;
+  }
 }
 
 static int upload_pair(DevMem &d64, DevMem &d32, const double *h, size_t n, hipStream_t s) {
@@ -230,7 +265,16 @@ extern "C" int d2mi_set_phase(d2mi_ctx *ctx, int phase, int metric, int dim, siz
   D2MI_TRY(p.p_str.assign(p_str, size * sizeof(int), s));
   D2MI_TRY(p.cum.assign(cum64.data(), size * sizeof(long long), s));
   D2MI_TRY(upload_pair(p.w64, p.w32, p_w, col, s));
-  if (metric == D2MI_EUCLIDEAN) D2MI_TRY(upload_pair(p.supp64, p.supp32, p_supp, col * dim, s));
+  if (metric == D2MI_EUCLIDEAN) {
+    D2MI_TRY(upload_pair(p.supp64, p.supp32, p_supp, col * dim, s));
+    D2MI_TRY(p.xmean.ensure(sizeof(double) * size * dim));
+    D2MI_TRY(p.xspread.ensure(sizeof(double) * size));
+    const unsigned blocks = (unsigned)std::min<size_t>((size + 255) / 256, 4096);
+    k_obj_moments<<<blocks, 256, 0, s>>>(size, dim, p.p_str.as<int>(), p.cum.as<long long>(),
+                                         p.supp64.as<double>(), p.xmean.as<double>(),
+                                         p.xspread.as<double>());
+    D2MI_HIP(hipGetLastError());
+  }
   if (p_supp_sym && metric != D2MI_EUCLIDEAN)
     D2MI_TRY(p.sym.assign(p_supp_sym, col * sizeof(int), s));
   if (metric == D2MI_WORD_EMBED)

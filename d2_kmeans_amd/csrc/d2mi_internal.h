@@ -6,6 +6,8 @@
 #include <cstdarg>
 #include <cstddef>
 #include <cstdint>
+#include <map>
+#include <tuple>
 #include <vector>
 
 #include "d2mi.h"
@@ -47,6 +49,8 @@ struct PhaseDev {
   DevMem cum;                   // int64[N]
   DevMem w64, w32;              // weights  double/float [col]
   DevMem supp64, supp32;        // supports double/float [col*dim] (Euclidean)
+  DevMem xmean, xspread;        // Euclidean: per object, FP64 mean support point [N*dim]
+                                // and sum of squared distances to it [N]
   DevMem sym;                   // int[col] word ids / histogram bins
   DevMem vocab64, vocab32;      // [V*dim]
   DevMem dist64, dist32;        // [V*V]
@@ -70,6 +74,18 @@ struct PhaseDev {
   DevMem bucket_map;
 };
 
+// Plan-kernel choice of one object-size class found by d2mi_badmm_tune.
+struct PlanTuneKey {
+  int prec, cm, dim, s, d, m_max;
+  bool operator<(const PlanTuneKey &o) const {
+    return std::tie(prec, cm, dim, s, d, m_max) < std::tie(o.prec, o.cm, o.dim, o.s, o.d, o.m_max);
+  }
+};
+struct PlanTuneVal {
+  int kind;     // kernel form (PlanKind in badmm.hip)
+  int wg_mode;  // workgroups per CU: 0 occupancy limit, 1 one less, 2 twice
+};
+
 struct d2mi_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -88,6 +104,17 @@ struct d2mi_ctx {
   DevMem el_l, el_u, el_s, el_c, el_r;  // double[N*K], double[N], double[K], double[K*K], char[N]
   DevMem dist, mask, jobs, lpstat, cnt, mind2, red;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  // B-ADMM inner loop as HIP graphs, per (phase, precision and cost kind,
+  // maxIters, updatePerLoops); pass_ev: the eager loop's per-pass events
+  int badmm_graph = -1;  // -1: environment decides, 0 off, 1 on
+  struct BadmmGraph {
+    hipGraphExec_t exec = nullptr;
+  };
+  std::map<std::tuple<int, int, int, int>, BadmmGraph> badmm_exec;
+  std::vector<hipEvent_t> pass_ev;
+  // plan-kernel tuner: choices per size class, plan copies taken while timing
+  std::map<PlanTuneKey, PlanTuneVal> plan_tune;
+  DevMem tuneZ, tuneY;
   size_t lds_max = 0;
   int num_cu = 0;
 };

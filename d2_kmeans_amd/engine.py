@@ -113,7 +113,25 @@ class Engine:
         tr = np.array([[st.trace[i][c] for c in range(4)] for i in range(st.n_trace)])
         return dict(rho=st.rho, obj=st.obj, trace=tr.reshape(-1, 4), ms_total=st.ms_total,
                     ms_plan=st.ms_plan, n_plan_launches=st.n_plan_launches,
-                    plan_elems=st.plan_elems)
+                    plan_elems=st.plan_elems, graph=bool(st.graph),
+                    plan_forms=st.plan_forms.decode(errors="replace"))
+
+    def set_graph(self, on: bool):
+        """Inner iterations of an update as one replayed HIP graph (default)
+        or launched one by one; results are identical."""
+        check(lib().d2mi_badmm_set_graph(self._h, 1 if on else 0))
+
+    def tune(self, idx: int, precision: int = 32, reps: int = 3) -> str:
+        """Times every feasible plan-kernel form and workgroups-per-CU
+        setting per object-size class on the resident plans (restored
+        afterwards) and keeps the fastest for later updates; returns the
+        report, one line per class."""
+        buf = C.create_string_buffer(8192)
+        check(lib().d2mi_badmm_tune(self._h, idx, precision, reps, buf, len(buf)))
+        return buf.value.decode(errors="replace")
+
+    def tune_clear(self):
+        check(lib().d2mi_badmm_tune_clear(self._h))
 
     def centroid_sphBregman(self, idx_ph: int, opts: BADMMOptions = None, precision: int = 32):
         """``d2_centroid_sphBregman(p_data, var_work, idx_ph, c0, c)`` with the

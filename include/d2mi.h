@@ -80,9 +80,13 @@ typedef struct {
   int n_trace;         /* residual rows recorded (centroid_Bregman.c:300-316) */
   double trace[64][4]; /* (iter+1, obj, primres, dualres) */
   double ms_total;     /* device time of the whole update (HIP events) */
-  double ms_plan;      /* summed device time of the plan-update kernel */
+  double ms_plan;      /* summed device time of the plan-update kernel; -1 when the
+                          update ran as a graph (HIP events recorded by graph nodes
+                          cannot be timed): d2mi_badmm_set_graph(ctx, 0) times it */
   int n_plan_launches; /* launches of the plan-update kernel */
   double plan_elems;   /* s * col, plan elements per inner iteration */
+  int graph;           /* 1: the inner iterations ran as one replayed HIP graph */
+  char plan_forms[200]; /* per object-size class: "m<=M:form/wgpcu(objects)" */
 } d2mi_badmm_stats;
 
 /* One B-ADMM centroid update of `phase` (d2_centroid_sphBregman,
@@ -93,6 +97,32 @@ typedef struct {
  * variant: D2MI_BADMM_ENGINE or D2MI_BADMM_PLAIN.  stats may be NULL. */
 int d2mi_badmm(d2mi_ctx *ctx, int phase, int maxIters, double rhoCoeff, int updatePerLoops,
                int precision, int variant, d2mi_badmm_stats *stats);
+
+/* The inner iterations of an update (every pass's plan kernels, the
+ * per-pass reductions and, for stored costs, the support update and cost
+ * rebuild) are captured into one HIP graph and replayed; the executable graph
+ * is kept per (phase, precision, maxIters, updatePerLoops) and refreshed in
+ * place with hipGraphExecUpdate.  on = 0 launches the same kernels one by one
+ * (results are identical) and times every pass (stats ms_plan).  Default: on
+ * (environment D2MI_BADMM_GRAPH=0 turns it off).  No counterpart in the
+ * reference, whose loop is host code (centroid_Bregman.c:150-320). */
+int d2mi_badmm_set_graph(d2mi_ctx *ctx, int on);
+
+/* Per object-size class autotuner of the plan kernel.  For the current
+ * labels and resident plans of `phase` at `precision` (one d2mi_badmm call
+ * must have run since the last upload), every feasible kernel form
+ * (register-resident columns, plan in LDS, general, HBM scratch) and
+ * workgroups-per-CU setting (the occupancy limit, one less, twice) is timed
+ * over `reps` inner passes per class; the fastest is kept in the context,
+ * keyed by (precision, cost kind, s, d, class size limit), and used by later
+ * d2mi_badmm calls.  The plans are saved before and restored after, so the
+ * call changes no result state; a later update may differ from an untuned
+ * one in the last bits (another summation order).  report (may be NULL)
+ * receives one line per class.  No counterpart in the reference. */
+int d2mi_badmm_tune(d2mi_ctx *ctx, int phase, int precision, int reps, char *report,
+                    size_t report_len);
+/* Drops every tuned choice of the context (rule-based forms again). */
+int d2mi_badmm_tune_clear(d2mi_ctx *ctx);
 
 /* Copies the resident plan (which: 0 = Z, 1 = Y) of `phase` at `precision`
  * to host memory as double[s*col]. */

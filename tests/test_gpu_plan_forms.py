@@ -119,3 +119,24 @@ def test_size_classes_large_objects(engine, precision):
         s64, c64, _ = update(engine, data, lab, c0, opts, 64)
         assert abs(sa["obj"] - s64["obj"]) <= 1e-5 * abs(s64["obj"])
         assert rel(ca.p_supp, c64.p_supp) < 1e-4
+
+
+@pytest.mark.parametrize("d", [2, 3, 16])
+def test_rho_from_object_moments_matches_direct_sum(engine, d):
+    """rho (centroid_Bregman.c:109) from each object's mean support point and
+    spread against the element-by-element ground-cost sum, FP64: the identity
+    sum_k sum_j |c_k - x_j|^2 = m sum_k |c_k - mu|^2 + s sum_j |x_j - mu|^2 is
+    exact, so the two agree to rounding (1e-13 relative), and both agree with
+    the oracle's rho to 1e-12."""
+    say(f"rho from object moments, d={d}")
+    data, _ = synth.gaussian_mixture(n=1500, d=d, m_lo=4, m_hi=30, seed=5 + d)
+    ph = data.phases[0]
+    c0 = O.centroid_rands(ph, 5, 7)
+    lab, lsw, _, _ = O.labeling(data, [c0])
+    opts = BADMMOptions(2, 2.0, 10)
+    sa, _, _ = update(engine, data, lab, c0, opts, 64)
+    sb, _, _ = update(engine, data, lab, c0, opts, 64, {"D2MI_RHO_DIRECT": "1"})
+    assert abs(sa["rho"] - sb["rho"]) <= 1e-13 * sb["rho"], (sa["rho"], sb["rho"])
+    co = c0.copy()
+    ro = O.badmm(ph, lab, np.ones(ph.size, np.int8), co, np.zeros(c0.s * ph.col), opts)
+    assert abs(sa["rho"] - ro["rho"]) <= 1e-12 * ro["rho"]
