@@ -23,8 +23,10 @@ Poisson arrivals at a fixed rate, served with continuous batching, with
 p50/p99 latency and queries/s beside the one-query-per-launch form on the
 same arrivals.  The ``word_embed`` object covers the word-embedding
 workload (V=50k, d=300): FP32 B-ADMM with MFMA cost tables gated against
-FP64, and the pdist2 GEMM against the FP32 matrix peak.  Prints ONE JSON line
-on stdout; progress goes to stderr.
+FP64, and the pdist2 GEMM against the FP32 matrix peak.  The ``multiphase``
+object covers 3-phase co-clustering (N=100k; Euclidean, dense and sparse
+histogram phases, K=32) phase by phase, each gated against FP64.  Prints ONE
+JSON line on stdout; progress goes to stderr.
 """
 from __future__ import annotations
 
@@ -245,6 +247,89 @@ def word_embed_leg(eng):
     return out
 
 
+def multiphase_leg(eng, args):
+    """Multi-phase co-clustering, batched: N objects of three phases
+    (Euclidean d=3 with 8-32 points; dense 64-bin histograms, cost (i-j)^2;
+    sparse 1024-bin histograms with 5-40 nonzeros, cost |i-j|), K=32, centroid
+    supports 16 / 64 / 1024 (a sparse-histogram centroid has s = V bins, as the
+    reference forces, clustering_io.c:182-185).  Per phase: the kernel forms
+    tuned, an FP64 and an FP32 update from the same state (objective gate),
+    then a timed FP32 update replayed as a graph and one launched eagerly for
+    the plan kernel's time.  Centroids start from the label-wise averages."""
+    from d2_kmeans_amd import synth
+    from d2_kmeans_amd.data import BADMMOptions, Centroids
+    K, S0 = 32, 16
+    opts = BADMMOptions(10, 2.0, 10)
+    t0 = time.perf_counter()
+    data, meta = synth.multiphase_histograms(n=args.multiphase_n)
+    log(f"[multiphase] generated N={args.multiphase_n} in {time.perf_counter() - t0:.1f} s")
+    labels = meta["classes"].astype(np.int32) % K
+    N = data.size
+    cents = []
+    for k, ph in enumerate(data.phases):
+        if k == 0:
+            cents.append(synth.seed_centroids(ph, K, S0, labels=labels))
+            continue
+        V = ph.vocab_size
+        H = np.zeros((K, V))
+        obj = np.repeat(np.arange(N), ph.p_str)
+        bins = (np.tile(np.arange(V), N) if ph.p_supp_sym is None else ph.p_supp_sym)
+        np.add.at(H, (labels[obj], bins), ph.p_w)
+        H += 1e-6
+        H /= H.sum(1, keepdims=True)
+        cents.append(Centroids(K, V, 0, H.reshape(-1)))
+    eng.set_data(data)
+    names = ["euclid_d3_s16", "dense_hist64", "sparse_hist1024"]
+    out = {"workload": f"3-phase co-clustering: N={N}, K={K}; phase 0 Euclidean d=3, m in [8,32], "
+                       "s=16; phase 1 dense 64-bin histograms, cost (i-j)^2, s=64; phase 2 sparse "
+                       "1024-bin histograms, 5-40 nonzeros, cost |i-j|, s=V=1024; labels = mixture "
+                       "component, centroids = label-wise averages",
+           "phases": {}}
+    t_total, upd_total = 0.0, 0
+    for k, ph in enumerate(data.phases):
+        c = cents[k]
+        elems = float(c.s) * float(ph.col)
+
+        def reset(lsw_all):
+            eng.set_centroids(k, c)
+            eng.set_labels(labels, np.ones(N, np.int8) if lsw_all else np.zeros(N, np.int8))
+
+        reset(True)
+        eng.badmm(k, opts, precision=32)
+        rep = eng.tune(k, 32, reps=2)
+        reset(True)
+        st64 = eng.badmm(k, opts, precision=64)
+        reset(True)
+        st32 = eng.badmm(k, opts, precision=32)
+        err = abs(st32["obj"] - st64["obj"]) / abs(st64["obj"])
+        reset(False)
+        eng.synchronize()
+        t1 = time.perf_counter()
+        stg = eng.badmm(k, opts, precision=32)
+        eng.synchronize()
+        dt = time.perf_counter() - t1
+        eng.set_graph(False)
+        reset(False)
+        ste = eng.badmm(k, opts, precision=32)
+        eng.set_graph(True)
+        plan_ms = ste["ms_plan"] / ste["n_plan_launches"]
+        t_total += dt
+        upd_total += N * (opts.maxIters + 1)
+        out["phases"][names[k]] = {
+            "plan_elements": int(elems), "ms_per_update": 1e3 * dt,
+            "plan_updates_per_s": N * (opts.maxIters + 1) / dt, "graph": stg["graph"],
+            "plan_forms": stg["plan_forms"], "plan_kernel_ms_per_pass": plan_ms,
+            "algorithmic_GBps": 16.0 * elems / (plan_ms / 1e3) / 1e9,
+            "obj_fp64": st64["obj"], "obj_fp32": st32["obj"], "obj_rel_err": err,
+            "tuner": rep.strip()}
+        log(f"[multiphase] phase {k}: {out['phases'][names[k]]}")
+    out["plan_updates_per_s"] = upd_total / t_total
+    out["ms_per_update_all_phases"] = 1e3 * t_total
+    out["max_obj_rel_err"] = max(v["obj_rel_err"] for v in out["phases"].values())
+    log(f"[multiphase] {out['plan_updates_per_s']:.4g} plan updates/s over the three phases")
+    return out
+
+
 def headline_leg(eng, args):
     """B-ADMM plan updates/s on the image-like workload, gated against FP64."""
     from d2_kmeans_amd import synth
@@ -375,6 +460,8 @@ def main():
                     help="skip the headline leg (for profiling the other legs alone)")
     ap.add_argument("--no-assign", action="store_true")
     ap.add_argument("--no-word-embed", action="store_true")
+    ap.add_argument("--no-multiphase", action="store_true")
+    ap.add_argument("--multiphase-n", type=int, default=100_000)
     ap.add_argument("--assign-n", type=int, default=3000)
     ap.add_argument("--assign-qps", type=float, default=1000.0)
     ap.add_argument("--assign-plain-n", type=int, default=300)
@@ -396,6 +483,7 @@ def main():
     h = None if args.skip_badmm else headline_leg(eng, args)
     assign = None if args.no_assign else assign_leg(eng, args)
     word = None if args.no_word_embed else word_embed_leg(eng)
+    multi = None if args.no_multiphase else multiphase_leg(eng, args)
 
     hd = h or {}
     ph = hd.get("ph")
@@ -429,6 +517,7 @@ def main():
         "cpu_baseline": hd.get("cpu"),
         "assign": assign,
         "word_embed": word,
+        "multiphase": multi,
     }
     print(json.dumps(out), flush=True)
     eng.close()

@@ -1197,6 +1197,207 @@ void k_badmm_plan_reg(PlanArgs<T> a) {
   }
 }
 
+// ------------------------------------------------------- tall register form
+//
+// Plans with many rows and few columns -- sparse-histogram phases, whose
+// centroids have s = V bins (up to 1024) while objects hold 5-40 nonzeros --
+// updated with one thread per row k and the object's columns j < MC in
+// registers: a workgroup of ceil(s / 64) waves holds the whole s x m plan, so
+// each object costs one HBM round trip (Z and Y read and written once), where
+// the HBM-scratch form sweeps its plan several times.  Row sums (_drnorm,
+// :169) are thread-local; column sums (_dcnorm, :158) reduce over each wave
+// by shuffles and across waves through LDS, in a fixed order (results are
+// reproducible).  Objects come in size classes of 8 columns (MC = 8 ... 40):
+// the column loops have no per-column branches -- columns past m load the
+// last column again (in bounds), are computed, masked out of the row sums and
+// never stored -- so the compiler batches each group's loads.
+constexpr int TALL_NT = 1024;  // rows per workgroup at most (s <= 1024)
+constexpr int TALL_NW = TALL_NT / 64;
+constexpr int TALL_MC32 = 40, TALL_MC64 = 16;  // most columns held in registers
+
+// Wave total into lane 63 with DPP row shifts and row broadcasts (GFX9
+// cross-lane moves, plain VALU: no LDS round trip per step).  Lanes shifted in
+// from outside a row read 0; after row_shr 1, 2, 4, 8 lane 15 of each row holds
+// the row's sum, row_bcast:15 (rows 1 and 3) and row_bcast:31 (rows 2 and 3)
+// carry it on to lane 63.  A fixed order: reproducible.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_f32(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL,
+                                                               ROWS, 0xf, true));
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_f64(double v) {
+  typedef int i2 __attribute__((ext_vector_type(2)));
+  i2 u = __builtin_bit_cast(i2, v);
+  u.x = __builtin_amdgcn_update_dpp(0, u.x, CTRL, ROWS, 0xf, true);
+  u.y = __builtin_amdgcn_update_dpp(0, u.y, CTRL, ROWS, 0xf, true);
+  return __builtin_bit_cast(double, u);
+}
+template <int CTRL, int ROWS, typename T>
+__device__ __forceinline__ T dpp_t(T v) {
+  if constexpr (std::is_same<T, double>::value) return dpp_f64<CTRL, ROWS>(v);
+  else return dpp_f32<CTRL, ROWS>(v);
+}
+template <typename T>
+__device__ __forceinline__ T wave_sum_to63(T v) {
+  v += dpp_t<0x111, 0xf>(v);  // row_shr:1
+  v += dpp_t<0x112, 0xf>(v);  // row_shr:2
+  v += dpp_t<0x114, 0xf>(v);  // row_shr:4
+  v += dpp_t<0x118, 0xf>(v);  // row_shr:8
+  v += dpp_t<0x142, 0xa>(v);  // row_bcast:15 into rows 1, 3
+  v += dpp_t<0x143, 0xc>(v);  // row_bcast:31 into rows 2, 3
+  return v;
+}
+
+// MODE 0: lean (no residual), MODE 1: first pass, MODE 2: residual pass
+// (anything).  Histogram metrics have no support update.
+template <typename T, int MC, int MODE>
+__global__ __launch_bounds__(TALL_NT) void k_badmm_plan_tall(PlanArgs<T> a) {
+  constexpr bool F64 = std::is_same<T, double>::value;
+  __shared__ T colpart[TALL_NW][MC];
+  __shared__ T scl[MC];
+  __shared__ double red[3][TALL_NW];
+  const int s = a.s, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int nw = blockDim.x >> 6;
+  const bool row = tid < s;
+  const int k = row ? tid : s - 1;  // lanes past row s repeat the last row, unstored
+  const bool first = MODE >= 1 && a.first, resid = MODE == 2 && a.resid;
+  const int gi = blockIdx.x;
+  const int p0 = gi * a.chunk, p1 = min(a.N, p0 + a.chunk);
+  const double rho = a.scal[0];
+  const T inv_rho = (T)a.scal[1];
+  const int voff = k * (int)sizeof(T), cstride = s * (int)sizeof(T);
+  // ground-cost table dist[id * V + k]: the row in the VGPR, the bin in an SGPR
+  const BufRes rdist = buf_res(a.dist);
+  const int vstride = a.V * (int)sizeof(T);
+  double acc = 0.0, r_obj = 0.0, r_pr = 0.0, r_dr = 0.0;
+  int cur = -1;
+
+  for (int p = p0; p < p1; ++p) {
+    const int o = const_load(a.perm + p);
+    const int l = const_load(a.label + o);
+    const int m = const_load(a.p_str + o);
+    const long long off = const_load(a.cum + o);
+    const size_t base = (size_t)s * off;
+    const bool reinit = first && (a.force_init || const_load(a.lsw + o) == 1);
+    if (l != cur) {  // no load in here: a branch join would wait on the plan loads
+      if (cur >= 0 && row) a.rec[(size_t)(gi + cur) * a.R + k] = acc;
+      acc = 0.0;
+      cur = l;
+    }
+    const T cwk = a.cw[(size_t)l * s + k];
+    const BufRes rz = buf_res(a.Z + base), ry = buf_res(a.Y + base);
+    T zr[MC], yr[MC];
+#pragma unroll
+    for (int c = 0; c < MC; ++c) {
+      const int cc = min(c, m - 1);
+      zr[c] = reinit ? (T)0 : buf_load<T>(rz, voff, cc * cstride);
+      yr[c] = first ? (T)0 : buf_load<T>(ry, voff, cc * cstride);
+    }
+
+    // pass 1 -- step 1a: t = Z exp(-(C + Y)) + 1e-9 (:155-157), C = dist[id_j][k],
+    // 8 columns at a time (a scheduling barrier between groups keeps every
+    // column's cost load from being hoisted, which would spill the plan)
+#pragma unroll
+    for (int c = 0; c < MC; ++c) {
+      const int cc = min(c, m - 1);
+      const int id = const_load(a.sym + off + cc);
+      const T cost = buf_load<T>(rdist, voff, id * vstride);
+      const T z = reinit ? const_load(a.w + off + cc) * cwk : zr[c];
+      zr[c] = z * plan_exp<T>(-(scale_cost<T>(cost, rho, inv_rho) + yr[c])) + (T)1e-9;
+      if (c % 8 == 7) __builtin_amdgcn_sched_barrier(0);
+    }
+    __syncthreads();  // every wave is done with the previous object's colpart, scl
+    // step 1b: column sums (_dcnorm, :158): wave totals, then over the waves
+#pragma unroll
+    for (int c = 0; c < MC; ++c) {
+      const T v = wave_sum_to63<T>(row ? zr[c] : (T)0);
+      if (lane == 63) colpart[wv][c] = v;
+    }
+    __syncthreads();
+    if (tid < MC) {
+      T sc = 0;
+      if (tid < m) {
+        T cs = 0;
+        for (int w = 0; w < nw; ++w) cs += colpart[w][tid];
+        sc = F64 ? cs : a.w[off + tid] / cs;
+      }
+      scl[tid] = sc;
+    }
+    __syncthreads();
+    // pass 3 -- step 1c: x = t / colsum * w (:159); step 2a: z' = x exp(Y) +
+    // 1e-9 (:166); the row sum (_drnorm, :169) is this thread's
+    T rk = 0;
+    T xr[MODE == 2 || F64 ? MC : 1];
+#pragma unroll
+    for (int c = 0; c < MC; ++c) {
+      T x;
+      if constexpr (F64) x = c < m ? (zr[c] / scl[c]) * const_load(a.w + off + min(c, m - 1)) : 0.0;
+      else x = zr[c] * scl[c];
+      const T zp = x * plan_exp<T>(yr[c]) + (T)1e-9;
+      if (c < m) rk += zp;
+      zr[c] = zp;
+      if constexpr (MODE == 2 || F64) xr[c] = x;
+      else yr[c] += x;
+      if (c % 8 == 7) __builtin_amdgcn_sched_barrier(0);
+    }
+    // step 4: this object's normalised row sums into the label's sums (:180-185)
+    const double rs = wave_sum_to63<double>(row ? (double)rk : 0.0);
+    if (lane == 63) red[0][wv] = rs;
+    __syncthreads();
+    double rtot = 0.0;
+    for (int w = 0; w < nw; ++w) rtot += red[0][w];
+    if (row) acc += (double)rk / rtot;
+    // pass 4 -- step 2c + 3: z = z' / rowsum * c_w (:169-170); Y += X - Z (:175)
+    const T gk = F64 ? rk : cwk / rk;
+#pragma unroll
+    for (int c = 0; c < MC; ++c) {
+      const T z = F64 ? (zr[c] / gk) * cwk : zr[c] * gk;
+      if constexpr (MODE == 2 || F64) {
+        if (resid && c < m) {  // step 6 (:300-305): Z before this pass, cost again
+          const T z0 = reinit ? const_load(a.w + off + c) * cwk : buf_load<T>(rz, voff, c * cstride);
+          const T cost = scale_cost<T>(
+              buf_load<T>(rdist, voff, const_load(a.sym + off + c) * vstride), rho, inv_rho);
+          if (row) {
+            r_obj += (double)cost * (double)xr[c];
+            r_pr += fabs((double)xr[c] - (double)z);
+            r_dr += fabs((double)z0 - (double)z);
+          }
+        }
+        yr[c] = yr[c] + (xr[c] - z);
+      } else {
+        yr[c] = yr[c] - z;
+      }
+      zr[c] = z;
+    }
+#pragma unroll
+    for (int c = 0; c < MC; ++c) asm volatile("" : "+v"(zr[c]), "+v"(yr[c]));
+#pragma unroll
+    for (int c = 0; c < MC; ++c) {
+      if (c < m && row) {
+        buf_store<T>(zr[c], rz, voff, c * cstride);
+        buf_store<T>(yr[c], ry, voff, c * cstride);
+      }
+    }
+  }
+  if (cur >= 0 && row) a.rec[(size_t)(gi + cur) * a.R + k] = acc;
+  if (resid) {
+    const double v0 = wave_sum(r_obj), v1 = wave_sum(r_pr), v2 = wave_sum(r_dr);
+    __syncthreads();
+    if (lane == 0) {
+      red[0][wv] = v0;
+      red[1][wv] = v1;
+      red[2][wv] = v2;
+    }
+    __syncthreads();
+    if (tid < 3) {
+      double t = 0.0;
+      for (int w = 0; w < nw; ++w) t += red[tid][w];
+      a.resid_part[gi * 3 + tid] = t;
+    }
+  }
+}
+
 // ------------------------------------------------------------- finalize
 
 constexpr int MAXB = 8;  // object-size buckets of one update
@@ -1983,9 +2184,9 @@ namespace {
 // resident workgroups per CU of a plan kernel at this LDS size (HIP's
 // occupancy calculator), so one launch is exactly one round of workgroups
 template <typename F>
-int occupancy_of(F kernel, size_t lds) {
+int occupancy_of(F kernel, size_t lds, int threads = NT) {
   int n = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, NT, lds) != hipSuccess || n <= 0)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, threads, lds) != hipSuccess || n <= 0)
     n = 1;
   return n;
 }
@@ -1995,7 +2196,7 @@ int occupancy_of(F kernel, size_t lds) {
 // (plan in LDS, s <= 64), k_badmm_plan (any s) or its HBM-scratch form (GM);
 // LEAN instances serve passes after the first with no support update,
 // residual or kept X
-enum PlanKind { PK_LEGACY, PK_GM, PK_64, PK_REG8, PK_REG16 };
+enum PlanKind { PK_LEGACY, PK_GM, PK_64, PK_REG8, PK_REG16, PK_TALL };
 
 const char *plan_kind_name(PlanKind k) {
   switch (k) {
@@ -2003,9 +2204,14 @@ const char *plan_kind_name(PlanKind k) {
     case PK_64: return "lds";
     case PK_REG8: return "reg8";
     case PK_REG16: return "reg16";
+    case PK_TALL: return "tall";
     default: return "general";
   }
 }
+
+inline bool is_tall(PlanKind k) { return k == PK_TALL; }
+// workgroup size of a plan kernel: one wave per 64 rows for the tall form
+inline int plan_threads(PlanKind k, int s) { return is_tall(k) ? (s + 63) / 64 * 64 : NT; }
 
 // d2mi_badmm_tune's request: passes timed per candidate, and the report
 struct TuneReq {
@@ -2023,9 +2229,50 @@ struct PlanKernels {
   static int reg_mode(const PlanArgs<T> &pa) {
     return LEAN ? 0 : pa.resid ? 2 : 1;
   }
+  // the tall form (sparse histograms, s > 64): MC = the size class's column
+  // count (a multiple of 8), passed as pa.m_max
+  static constexpr bool TALL = CM == COST_SPARSE && DIM == 0;
+  static constexpr int TALL_MC = F32 ? TALL_MC32 : TALL_MC64;
+  template <int MC>
+  static int tall_occ1(int mode, int threads) {
+    if constexpr (TALL && MC <= TALL_MC) {
+      if (mode == 0) return occupancy_of(k_badmm_plan_tall<T, MC, 0>, 0, threads);
+      if (mode == 2) return occupancy_of(k_badmm_plan_tall<T, MC, 2>, 0, threads);
+      return occupancy_of(k_badmm_plan_tall<T, MC, 1>, 0, threads);
+    }
+    return 1;
+  }
+  template <int MC>
+  static void tall_run1(int G, hipStream_t S, const PlanArgs<T> &pa, int mode) {
+    if constexpr (TALL && MC <= TALL_MC) {
+      const int th = plan_threads(PK_TALL, pa.s);
+      if (mode == 0) k_badmm_plan_tall<T, MC, 0><<<G, th, 0, S>>>(pa);
+      else if (mode == 2) k_badmm_plan_tall<T, MC, 2><<<G, th, 0, S>>>(pa);
+      else k_badmm_plan_tall<T, MC, 1><<<G, th, 0, S>>>(pa);
+    }
+  }
+  static int tall_occ(int mc, int mode, int threads) {
+    switch (mc) {
+      case 8: return tall_occ1<8>(mode, threads);
+      case 16: return tall_occ1<16>(mode, threads);
+      case 24: return tall_occ1<24>(mode, threads);
+      case 32: return tall_occ1<32>(mode, threads);
+      default: return tall_occ1<40>(mode, threads);
+    }
+  }
+  static void tall_run(int G, hipStream_t S, const PlanArgs<T> &pa, int mode) {
+    switch (pa.m_max) {
+      case 8: tall_run1<8>(G, S, pa, mode); break;
+      case 16: tall_run1<16>(G, S, pa, mode); break;
+      case 24: tall_run1<24>(G, S, pa, mode); break;
+      case 32: tall_run1<32>(G, S, pa, mode); break;
+      default: tall_run1<40>(G, S, pa, mode);
+    }
+  }
   template <bool LEAN>
-  static int occ(size_t lds, PlanKind k, int mode) {
+  static int occ(size_t lds, PlanKind k, int mode, int s, int mc) {
     switch (k) {
+      case PK_TALL: return tall_occ(mc, LEAN ? 0 : mode, plan_threads(k, s));
       case PK_REG16:
         if constexpr (F32) {
           if constexpr (LEAN) return occupancy_of(k_badmm_plan_reg<T, CM, DIM, 16, 0>, lds);
@@ -2046,6 +2293,7 @@ struct PlanKernels {
   static void run(int G, size_t lds, hipStream_t S, const PlanArgs<T> &pa, PlanKind k) {
     const int mode = reg_mode<LEAN>(pa);
     switch (k) {
+      case PK_TALL: tall_run(G, S, pa, mode); break;
       case PK_REG16:
         if constexpr (F32) {
           if constexpr (LEAN) k_badmm_plan_reg<T, CM, DIM, 16, 0><<<G, NT, lds, S>>>(pa);
@@ -2068,10 +2316,10 @@ struct PlanKernels {
                                                                  pa.label, pa.lsw);
     }
   }
-  static int occupancy(size_t lds, PlanKind k, bool lean, bool resid) {
+  static int occupancy(size_t lds, PlanKind k, bool lean, bool resid, int s, int mc) {
     if constexpr (F32)
-      if (lean && k != PK_GM) return occ<true>(lds, k, 0);
-    return occ<false>(lds, k, resid ? 2 : 1);
+      if (lean && k != PK_GM) return occ<true>(lds, k, 0, s, mc);
+    return occ<false>(lds, k, resid ? 2 : 1, s, mc);
   }
   static void launch(int G, size_t lds, hipStream_t S, const PlanArgs<T> &pa, PlanKind k,
                      bool lean) {
@@ -2352,11 +2600,17 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
     // classes end at the register form's limit (16 columns per thread in
     // FP32, 8 in FP64), then twice, four and eight times that, then m_max.
     // D2MI_PLAN_NO_BUCKETS=1 launches all objects together.
+    // For sparse histograms with 64 < s <= 1024 (tall plans) the classes end
+    // at the tall register form's column counts (16, and 40 in FP32).
     std::vector<int> lims;
     if (s <= 64 && !getenv_flag("D2MI_PLAN_NO_BUCKETS")) {
       const int cpi = NT / s;
       for (int c = F64 ? 8 : 16; c * cpi < m_max && (int)lims.size() < 4; c *= 2)
         lims.push_back(c * cpi);
+    }
+    if (CM == COST_SPARSE && s > 64 && s <= TALL_NT && !getenv_flag("D2MI_PLAN_NO_BUCKETS")) {
+      const int mc_max = F64 ? TALL_MC64 : TALL_MC32;
+      for (int c = 8; c <= mc_max && c < m_max; c += 8) lims.push_back(c);
     }
     lims.push_back(m_max);
     const int nbk = (int)lims.size();
@@ -2415,6 +2669,9 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
     auto fits = [&](int m_lim, PlanKind k) -> bool {
       const int ncols = s <= 64 ? (m_lim + NT / s - 1) / (NT / s) : 1 << 30;
       switch (k) {
+        case PK_TALL:  // class limits are multiples of 8 (the kernel's column slots)
+          return CM == COST_SPARSE && s > 64 && s <= TALL_NT && m_lim % 8 == 0 &&
+                 m_lim <= (F64 ? TALL_MC64 : TALL_MC32);
         case PK_REG16:
           if (F64) return false;
           [[fallthrough]];
@@ -2439,7 +2696,10 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
       B.m_max = lims[b];
       B.wg_mode = 0;
       PlanKind k = PK_GM;
-      if (!force_gm) {
+      const bool no_tall = getenv_flag("D2MI_PLAN_NO_TALL");
+      if (!force_gm && !no_tall && fits(B.m_max, PK_TALL)) {
+        k = PK_TALL;
+      } else if (!force_gm) {
         if (!no_64 && fits(B.m_max, PK_64)) {
           k = PK_64;
           if (!no_reg && fits(B.m_max, PK_REG8)) k = PK_REG8;
@@ -2459,6 +2719,7 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
       any_gm = any_gm || B.gm;
     }
     auto lds_of = [&](const Bucket &B, int keepx, int R) -> size_t {
+      if (is_tall(B.kind)) return 0;  // static LDS only
       if (B.kind == PK_REG8 || B.kind == PK_REG16)
         return PlanRegLds(sizeof(T), B.sp, B.m_max, s, d, R).total;
       return B.k64 ? Plan64Lds(sizeof(T), B.sp, B.m_max, s, d, keepx, R).total
@@ -2466,10 +2727,10 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
     };
     auto occupancy = [&](const Bucket &B, size_t lds, bool lean, bool resid) {
       if constexpr (CM == COST_EUCLID) {
-        if (dim_sel == 2) return PlanKernels<T, CM, 2>::occupancy(lds, B.kind, lean, resid);
-        if (dim_sel == 3) return PlanKernels<T, CM, 3>::occupancy(lds, B.kind, lean, resid);
+        if (dim_sel == 2) return PlanKernels<T, CM, 2>::occupancy(lds, B.kind, lean, resid, s, B.m_max);
+        if (dim_sel == 3) return PlanKernels<T, CM, 3>::occupancy(lds, B.kind, lean, resid, s, B.m_max);
       }
-      return PlanKernels<T, CM, 0>::occupancy(lds, B.kind, lean, resid);
+      return PlanKernels<T, CM, 0>::occupancy(lds, B.kind, lean, resid, s, B.m_max);
     };
     auto per_cu_of = [&](const Bucket &B, size_t lds, bool lean, bool resid) {
       if (wg_env > 0) return wg_env;
@@ -2534,7 +2795,7 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
         // than 3% faster on the plan passes, since more workgroups also cost
         // the per-pass reduction (more records) that these timings leave out
         std::vector<PlanTuneVal> cands{PlanTuneVal{(int)B.kind, 0}};
-        const PlanKind kinds[] = {PK_REG8, PK_REG16, PK_64, PK_LEGACY, PK_GM};
+        const PlanKind kinds[] = {PK_TALL, PK_REG8, PK_REG16, PK_64, PK_LEGACY, PK_GM};
         bool any_fast = false;
         for (PlanKind k : kinds)
           if (k != PK_GM && fits(B.m_max, k)) any_fast = true;

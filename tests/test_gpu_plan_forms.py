@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 from d2_kmeans_amd import synth
-from d2_kmeans_amd.data import BADMMOptions
+from d2_kmeans_amd.data import BADMMOptions, Centroids, D2Data
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -140,3 +140,34 @@ def test_rho_from_object_moments_matches_direct_sum(engine, d):
     co = c0.copy()
     ro = O.badmm(ph, lab, np.ones(ph.size, np.int8), co, np.zeros(c0.s * ph.col), opts)
     assert abs(sa["rho"] - ro["rho"]) <= 1e-12 * ro["rho"]
+
+
+@pytest.mark.parametrize("V", [200, 1024])
+def test_tall_form_sparse_histograms(engine, V):
+    """Sparse histograms with s = V bins and 5-40 nonzeros: the tall register
+    form (one thread per bin row, size classes of 8 columns) against the
+    HBM-scratch form in FP32 (objective within 1e-5, weights within 1e-4:
+    different summation orders) and against the FP64 oracle (1e-9).  V = 200
+    leaves the last wave partly idle."""
+    say(f"tall form: sparse histograms, V={V}")
+    data, meta = synth.multiphase_histograms(n=300, V2=V, seed=23 + V)
+    ph = data.phases[2]
+    one = D2Data([ph])
+    K = 3
+    rng = np.random.default_rng(V)
+    c0 = Centroids(K, V, 0, rng.dirichlet(np.ones(V), size=K).reshape(-1))
+    lab = (meta["classes"] % K).astype(np.int32)
+    opts = BADMMOptions(10, 2.0, 10)
+    st_t, c_t, _ = update(engine, one, lab, c0, opts, 32)
+    assert "tall" in st_t["plan_forms"], st_t["plan_forms"]
+    st_h, c_h, _ = update(engine, one, lab, c0, opts, 32, {"D2MI_PLAN_NO_TALL": "1"})
+    assert "tall" not in st_h["plan_forms"]
+    assert abs(st_t["obj"] - st_h["obj"]) <= 1e-5 * abs(st_h["obj"])
+    assert np.max(np.abs(c_t.p_w - c_h.p_w)) <= 1e-4 * np.max(c_h.p_w)
+    st64, c64, _ = update(engine, one, lab, c0, opts, 64)
+    assert "tall" in st64["plan_forms"], st64["plan_forms"]
+    co = c0.copy()
+    ro = O.badmm(ph, lab, np.ones(ph.size, np.int8), co, np.zeros(c0.s * ph.col), opts)
+    assert rel(c64.p_w, co.p_w) < 1e-9
+    assert abs(st64["obj"] - ro["obj"]) <= 1e-9 * abs(ro["obj"])
+    assert abs(st_t["obj"] - ro["obj"]) <= 1e-5 * abs(ro["obj"])
