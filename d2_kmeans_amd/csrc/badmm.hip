@@ -54,9 +54,48 @@ __device__ __forceinline__ float dexp<float>(float x) { return expf(x); }
 template <>
 __device__ __forceinline__ double dexp<double>(double x) { return exp(x); }
 
-__device__ __forceinline__ double wave_sum(double v) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+// Wave total into lane 63 with DPP row shifts and row broadcasts (GFX9
+// cross-lane moves, plain VALU: no LDS round trip per step).  Lanes shifted in
+// from outside a row read 0; after row_shr 1, 2, 4, 8 lane 15 of each row holds
+// the row's sum, row_bcast:15 (rows 1 and 3) and row_bcast:31 (rows 2 and 3)
+// carry it on to lane 63.  A fixed order: reproducible.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_f32(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL,
+                                                               ROWS, 0xf, true));
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_f64(double v) {
+  typedef int i2 __attribute__((ext_vector_type(2)));
+  i2 u = __builtin_bit_cast(i2, v);
+  u.x = __builtin_amdgcn_update_dpp(0, u.x, CTRL, ROWS, 0xf, true);
+  u.y = __builtin_amdgcn_update_dpp(0, u.y, CTRL, ROWS, 0xf, true);
+  return __builtin_bit_cast(double, u);
+}
+template <int CTRL, int ROWS, typename T>
+__device__ __forceinline__ T dpp_t(T v) {
+  if constexpr (std::is_same<T, double>::value) return dpp_f64<CTRL, ROWS>(v);
+  else return dpp_f32<CTRL, ROWS>(v);
+}
+template <typename T>
+__device__ __forceinline__ T wave_sum_to63(T v) {
+  v += dpp_t<0x111, 0xf>(v);  // row_shr:1
+  v += dpp_t<0x112, 0xf>(v);  // row_shr:2
+  v += dpp_t<0x114, 0xf>(v);  // row_shr:4
+  v += dpp_t<0x118, 0xf>(v);  // row_shr:8
+  v += dpp_t<0x142, 0xa>(v);  // row_bcast:15 into rows 1, 3
+  v += dpp_t<0x143, 0xc>(v);  // row_bcast:31 into rows 2, 3
   return v;
+}
+
+// Wave total of v on every lane (all 64 lanes must be active): the DPP sum
+// into lane 63, read back as a scalar.  Fixed order: reproducible.
+__device__ __forceinline__ double wave_sum(double v) {
+  typedef int i2 __attribute__((ext_vector_type(2)));
+  i2 u = __builtin_bit_cast(i2, wave_sum_to63<double>(v));
+  u.x = __builtin_amdgcn_readlane(u.x, 63);
+  u.y = __builtin_amdgcn_readlane(u.y, 63);
+  return __builtin_bit_cast(double, u);
 }
 
 // Block-wide sum with a fixed association order; every thread gets the
@@ -1214,40 +1253,6 @@ void k_badmm_plan_reg(PlanArgs<T> a) {
 constexpr int TALL_NT = 1024;  // rows per workgroup at most (s <= 1024)
 constexpr int TALL_NW = TALL_NT / 64;
 constexpr int TALL_MC32 = 40, TALL_MC64 = 16;  // most columns held in registers
-
-// Wave total into lane 63 with DPP row shifts and row broadcasts (GFX9
-// cross-lane moves, plain VALU: no LDS round trip per step).  Lanes shifted in
-// from outside a row read 0; after row_shr 1, 2, 4, 8 lane 15 of each row holds
-// the row's sum, row_bcast:15 (rows 1 and 3) and row_bcast:31 (rows 2 and 3)
-// carry it on to lane 63.  A fixed order: reproducible.
-template <int CTRL, int ROWS>
-__device__ __forceinline__ float dpp_f32(float v) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL,
-                                                               ROWS, 0xf, true));
-}
-template <int CTRL, int ROWS>
-__device__ __forceinline__ double dpp_f64(double v) {
-  typedef int i2 __attribute__((ext_vector_type(2)));
-  i2 u = __builtin_bit_cast(i2, v);
-  u.x = __builtin_amdgcn_update_dpp(0, u.x, CTRL, ROWS, 0xf, true);
-  u.y = __builtin_amdgcn_update_dpp(0, u.y, CTRL, ROWS, 0xf, true);
-  return __builtin_bit_cast(double, u);
-}
-template <int CTRL, int ROWS, typename T>
-__device__ __forceinline__ T dpp_t(T v) {
-  if constexpr (std::is_same<T, double>::value) return dpp_f64<CTRL, ROWS>(v);
-  else return dpp_f32<CTRL, ROWS>(v);
-}
-template <typename T>
-__device__ __forceinline__ T wave_sum_to63(T v) {
-  v += dpp_t<0x111, 0xf>(v);  // row_shr:1
-  v += dpp_t<0x112, 0xf>(v);  // row_shr:2
-  v += dpp_t<0x114, 0xf>(v);  // row_shr:4
-  v += dpp_t<0x118, 0xf>(v);  // row_shr:8
-  v += dpp_t<0x142, 0xa>(v);  // row_bcast:15 into rows 1, 3
-  v += dpp_t<0x143, 0xc>(v);  // row_bcast:31 into rows 2, 3
-  return v;
-}
 
 // MODE 0: lean (no residual), MODE 1: first pass, MODE 2: residual pass
 // (anything).  Histogram metrics have no support update.
