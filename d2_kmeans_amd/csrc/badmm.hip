@@ -1822,17 +1822,21 @@ __global__ __launch_bounds__(NT) void k_badmm_supp_fin(int s, int d, int chunk,
                                                        const double *__restrict__ rx,
                                                        double *__restrict__ cs64,
                                                        T *__restrict__ csT, int *flags) {
-  const int l = blockIdx.x;
+  // grid (ceil(s d / NT), K): one thread per (label, support coordinate); each
+  // sums its label's workgroup records in workgroup order
+  const int l = blockIdx.y;
   const int lo = seg[l], hi = seg[l + 1];
   const int glo = lo / chunk, ghi = (hi - 1) / chunk;
   const size_t R = (size_t)s * d;
-  for (size_t q = threadIdx.x; q < R; q += NT) {
+  const size_t q = (size_t)blockIdx.x * NT + threadIdx.x;
+  if (q < R) {
     double v = 0.0;
+#pragma unroll 4
     for (int g = glo; g <= ghi; ++g) v += rec[(size_t)(g + l) * R + q];
     const double r = rx[(size_t)l * s + q / d];
     if (!(r > 0.0)) {
       atomicOr(flags, FLAG_NUMERIC);
-      continue;
+      return;
     }
     const double c = v / r;
     cs64[(size_t)l * R + q] = c;
@@ -1924,6 +1928,39 @@ __global__ __launch_bounds__(NT) void k_badmm_rho_euclid(int N, int s, int d,
         a += u * u;
       }
     v += (double)p_str[o] * a + (double)s * spread[o];
+  }
+  v = block_sum(v, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = v;
+}
+
+// Ground-cost sums of a histogram phase (rho, :109) without its s x m costs.
+// Sparse histograms: object i's cost rows are dist[id_j * V + k], k < s, so
+// its sum is sum_j L[id_j] with L[v] = sum_{k<s} |dist[v V + k]|.  Dense
+// histograms: cost (k, j) is dist[k + j s], so the sum is sum_{j<m} L[j] with
+// L[j] = sum_{k<s} |dist[k + j s]|.  FP64, from the FP64 table.
+__global__ __launch_bounds__(NT) void k_hist_line_sums(const double *__restrict__ dist, int V,
+                                                       int s, int nlines, int sparse,
+                                                       double *__restrict__ L) {
+  const int v = blockIdx.x * NT + threadIdx.x;
+  if (v >= nlines) return;
+  double a = 0.0;
+  for (int k = 0; k < s; ++k) a += fabs(sparse ? dist[(size_t)v * V + k] : dist[k + (size_t)v * s]);
+  L[v] = a;
+}
+
+__global__ __launch_bounds__(NT) void k_badmm_rho_hist(int N, const int *__restrict__ p_str,
+                                                       const long long *__restrict__ cum,
+                                                       const int *__restrict__ sym, int sparse,
+                                                       const double *__restrict__ L,
+                                                       double *__restrict__ part) {
+  __shared__ double red[8];
+  double v = 0.0;
+  for (int o = blockIdx.x * NT + threadIdx.x; o < N; o += gridDim.x * NT) {
+    const int m = p_str[o];
+    const long long off = cum[o];
+    double a = 0.0;
+    for (int j = 0; j < m; ++j) a += L[sparse ? sym[off + j] : j];
+    v += a;
   }
   v = block_sum(v, red);
   if (threadIdx.x == 0) part[blockIdx.x] = v;
@@ -2418,7 +2455,8 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
     } else {
       k_badmm_supp_num<T, false><<<G, NT, 0, S>>>(sa2);
     }
-    k_badmm_supp_fin<T><<<K, NT, 0, S>>>(s, dsup, tab_chunk, seg, ctx->rec2.as<double>(),
+    k_badmm_supp_fin<T><<<dim3((unsigned)((s * dsup + NT - 1) / NT), K), NT, 0, S>>>(
+        s, dsup, tab_chunk, seg, ctx->rec2.as<double>(),
                                          ctx->rxbuf.as<double>(), p->cs64.as<double>(), csT, flags);
     D2MI_HIP(hipGetLastError());
     return D2MI_OK;
@@ -2483,6 +2521,18 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
                                               ctx->label.as<int>(), p->xmean.as<double>(),
                                               p->xspread.as<double>(), p->cs64.as<double>(),
                                               ctx->part.as<double>());
+    } else if (variant == D2MI_BADMM_ENGINE && (CM == COST_SPARSE || CM == COST_HIST) &&
+               !getenv_flag("D2MI_RHO_DIRECT")) {
+      const bool sparse = CM == COST_SPARSE;
+      const int nlines = sparse ? p->V : m_max;
+      D2MI_TRY(ctx->rowsum.ensure(sizeof(double) * (size_t)nlines));
+      k_hist_line_sums<<<(nlines + NT - 1) / NT, NT, 0, S>>>(p->dist64.as<double>(), p->V, s,
+                                                             nlines, sparse ? 1 : 0,
+                                                             ctx->rowsum.as<double>());
+      nparts = std::max(1, std::min((N + NT - 1) / NT, ctx->num_cu * 4));
+      k_badmm_rho_hist<<<nparts, NT, 0, S>>>(N, p->p_str.as<int>(), p->cum.as<long long>(), sym,
+                                             sparse ? 1 : 0, ctx->rowsum.as<double>(),
+                                             ctx->part.as<double>());
     } else {
       k_badmm_setup<T, CM><<<setup_blocks, NT, lds, S>>>(sa);
     }

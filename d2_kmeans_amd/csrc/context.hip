@@ -113,7 +113,8 @@ extern "C" int d2mi_destroy(d2mi_ctx *ctx) {
                    &ctx->el_u, &ctx->el_s, &ctx->el_c, &ctx->el_r, &ctx->dist, &ctx->mask,
                    &ctx->jobs, &ctx->lpstat, &ctx->cnt, &ctx->mind2, &ctx->red,
                    &ctx->rec2, &ctx->rxbuf, &ctx->bkey, &ctx->permk, &ctx->blkcnt2,
-                   &ctx->segk, &ctx->wtab, &ctx->wnorm, &ctx->tuneZ, &ctx->tuneY};
+                   &ctx->segk, &ctx->wtab, &ctx->wnorm, &ctx->tuneZ, &ctx->tuneY,
+                   &ctx->rowsum};
   for (DevMem *m : all) m->release();
   for (auto &e : ctx->ev)
     if (e) (void)hipEventDestroy(e);

@@ -97,6 +97,7 @@ struct d2mi_ctx {
   DevMem rec2, rxbuf;           // support-numerator records, X row sums per label
   DevMem bkey, permk, blkcnt2, segk;  // B-ADMM order by (object-size bucket, label)
   DevMem wtab, wnorm;           // word embeddings: pdist2(vocab, centroid supports), norms
+  DevMem rowsum;                // histogram phases: ground-cost line sums for rho
   // pdist2 scratch
   DevMem pA, pB, pC;
   // labeling / Elkan state (clustering.h:106-112), resident in HBM

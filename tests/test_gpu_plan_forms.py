@@ -171,3 +171,26 @@ def test_tall_form_sparse_histograms(engine, V):
     assert rel(c64.p_w, co.p_w) < 1e-9
     assert abs(st64["obj"] - ro["obj"]) <= 1e-9 * abs(ro["obj"])
     assert abs(st_t["obj"] - ro["obj"]) <= 1e-5 * abs(ro["obj"])
+
+
+@pytest.mark.parametrize("which", [1, 2])
+def test_rho_from_line_sums_histograms(engine, which):
+    """rho of a dense (1) or sparse (2) histogram phase from the ground-cost
+    table's line sums against the element-by-element sum, FP64: the same
+    terms in another order, so they agree to 1e-13 relative; and the update
+    still matches the oracle's rho to 1e-12."""
+    say(f"rho from line sums, histogram phase {which}")
+    data, meta = synth.multiphase_histograms(n=250, V2=300, seed=41 + which)
+    ph = data.phases[which]
+    one = D2Data([ph])
+    V, K = ph.vocab_size, 3
+    rng = np.random.default_rng(which)
+    c0 = Centroids(K, V, 0, rng.dirichlet(np.ones(V), size=K).reshape(-1))
+    lab = (meta["classes"] % K).astype(np.int32)
+    opts = BADMMOptions(1, 2.0, 10)
+    sa, _, _ = update(engine, one, lab, c0, opts, 64)
+    sb, _, _ = update(engine, one, lab, c0, opts, 64, {"D2MI_RHO_DIRECT": "1"})
+    assert abs(sa["rho"] - sb["rho"]) <= 1e-13 * sb["rho"], (sa["rho"], sb["rho"])
+    co = c0.copy()
+    ro = O.badmm(ph, lab, np.ones(ph.size, np.int8), co, np.zeros(c0.s * ph.col), opts)
+    assert abs(sa["rho"] - ro["rho"]) <= 1e-12 * ro["rho"]
