@@ -147,9 +147,10 @@ def assign_leg(eng, args):
     rng = np.random.default_rng(synth.SEED_BASE + 42)
     arrivals = np.cumsum(rng.exponential(1.0 / args.assign_qps, qph.size))
     # warm up both forms (kernels loaded, streams and staging created)
-    srv = Server(eng, 0, max_batch=128, streams=4)
+    srv = Server(eng, 0, max_batch=args.assign_batch, streams=args.assign_streams)
     srv.assign(Q[:64])
-    log(f"[assign] continuous batching, {qph.size} queries at {args.assign_qps} q/s")
+    log(f"[assign] continuous batching ({args.assign_batch} per group, {args.assign_streams} "
+        f"streams), {qph.size} queries at {args.assign_qps} q/s")
     lab, dist, lat, el = srv.replay(Q, arrivals)
     st = srv.stats()
     t0 = time.perf_counter()
@@ -178,6 +179,7 @@ def assign_leg(eng, args):
                     "B-ADMM, Elkan labeling)",
         "train_seconds": train_s,
         "queries": int(qph.size), "qps_offered": args.assign_qps,
+        "max_batch": args.assign_batch, "streams": args.assign_streams,
         "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
         "qps_achieved": qph.size / el, "burst_qps": burst,
         "lps_per_query": st["lps"] / max(st["completed"], 1), "groups": st["groups"],
@@ -464,6 +466,8 @@ def main():
     ap.add_argument("--multiphase-n", type=int, default=100_000)
     ap.add_argument("--assign-n", type=int, default=3000)
     ap.add_argument("--assign-qps", type=float, default=1000.0)
+    ap.add_argument("--assign-batch", type=int, default=48)
+    ap.add_argument("--assign-streams", type=int, default=6)
     ap.add_argument("--assign-plain-n", type=int, default=300)
     ap.add_argument("--assign-check", type=int, default=100)
     ap.add_argument("--assign-train", type=int, default=4000)

@@ -5,12 +5,12 @@
 // the centroid with the smallest exact transport distance (first minimum on
 // ties) and that distance, sqrt of the LP value (clustering.c:126-127).
 //
-// Continuous batching.  Queries wait in a host queue.  Whenever a stream of
-// a waiting query's size class (m <= 64, <= 192, larger; with three or more
-// streams each class has its own) is free and the HBM arena has room, that
-// class is admitted as one launch group: up to max_batch waiting queries of
-// the class, in arrival order, whose inputs, bounds and results fit the
-// arena.  The group's work runs on
+// Continuous batching.  Queries wait in a host queue.  Whenever a stream that
+// serves a waiting query's size class (m <= 64, <= 192, larger; with three or
+// more streams each class has one of its own and the rest are shared) is free
+// and the HBM arena has room, that class is admitted as one launch group: up
+// to max_batch waiting queries of the class, in arrival order, whose inputs,
+// bounds and results fit the arena.  The group's work runs on
 // its own stream, so groups admitted at different times overlap on the
 // device; finished groups free their stream and memory at the next progress
 // call.  Per group:
@@ -331,16 +331,17 @@ void arena_free(d2mi_server *S, size_t off, size_t bytes) {
 
 int size_class(int m) { return m <= 64 ? 0 : (m <= 192 ? 1 : 2); }
 
-// Streams are split by size class when there are at least three: stream 0
-// runs the largest queries, stream 1 the middle class, the rest the small
-// ones, so a query only ever waits behind groups of its own class (the
-// device still runs all of them at once).
+// With at least three streams, each size class has one of its own (stream 0
+// the largest queries, 1 the middle class, 2 the small ones), so no class can
+// be locked out by another; streams 3 and up are shared and go to the class of
+// the oldest waiting query.  A group only holds queries of one class (its
+// latency is its slowest LP), and a stream takes a new group only when its
+// last one is done, so a group never queues behind another on its stream.
+// Admission tries a class's own stream before the shared ones.
 bool slot_serves(const d2mi_server *S, int slot, int cls) {
   const int n = (int)S->slots.size();
-  if (n < 3) return true;
-  if (slot == 0) return cls == 2;
-  if (slot == 1) return cls == 1;
-  return cls == 0;
+  if (n < 3 || slot >= 3) return true;
+  return slot == 2 - cls;
 }
 
 // device bytes of a group of R queries with `col` support points in total
@@ -593,8 +594,8 @@ extern "C" int d2mi_serve_open(d2mi_ctx *ctx, int phase, const d2mi_serve_opts *
   S->s = P->s;
   S->d = P->dim;
   d2mi_serve_opts o{};
-  o.max_batch = 128;
-  o.streams = 4;
+  o.max_batch = 48;
+  o.streams = 6;
   o.hbm_bytes = (size_t)1 << 30;
   o.max_m = 4096;
   o.use_triangle = 1;

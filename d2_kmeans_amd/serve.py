@@ -24,7 +24,7 @@ class Server:
     ``plain=True`` serves one query per launch, in arrival order, with one
     group in flight (the one-request-per-launch form)."""
 
-    def __init__(self, engine, phase: int = 0, max_batch: int = 128, streams: int = 4,
+    def __init__(self, engine, phase: int = 0, max_batch: int = 48, streams: int = 6,
                  hbm_bytes: int = 1 << 30, max_m: int = 2048, use_triangle: bool = True,
                  plain: bool = False):
         if plain:

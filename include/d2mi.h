@@ -220,8 +220,10 @@ int d2mi_clustering(d2mi_ctx *ctx, int K, int max_iter, int selected_phase, int 
 typedef struct d2mi_server d2mi_server;
 
 typedef struct {
-  int max_batch;     /* queries per launch group (0: 128) */
-  int streams;       /* launch groups in flight at once (0: 4, at most 16) */
+  int max_batch;     /* queries per launch group (0: 48) */
+  int streams;       /* launch groups in flight at once (0: 6, at most 16); with 3 or
+                        more, streams 0-2 are kept for the large, middle and small size
+                        class and the rest go to the class of the oldest waiting query */
   size_t hbm_bytes;  /* device memory for admitted queries (0: 1 GiB) */
   int max_m;         /* largest query accepted (0 or above 2048: 2048 support points) */
   int use_triangle;  /* 1: also prune with W2(c1, c) > 2 W2(x, c1) (Elkan's test) */
