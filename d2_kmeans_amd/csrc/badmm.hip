@@ -23,6 +23,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <string>
 #include <type_traits>
 
@@ -909,11 +910,13 @@ __device__ __forceinline__ void buf_store(T v, BufRes r, int voff, int soff) {
 // sums); column stride sp even, as for k_badmm_plan64.
 struct PlanRegLds {
   size_t A, col, w, xs, cs, cw, g, rp, ids, acc, red, up, total;
-  __host__ __device__ PlanRegLds(int tsize, int sp, int m_max, int s, int d, int R) {
+  // noA: t lives in the prefetch form's static staging array instead
+  __host__ __device__ PlanRegLds(int tsize, int sp, int m_max, int s, int d, int R,
+                                 bool noA = false) {
     const size_t dd = d > 0 ? d : 1;
     const int cpi = NT / s;
     size_t o = 0;
-    A = o; o = align16(o + (size_t)sp * m_max * tsize);
+    A = o; o = align16(o + (noA ? 0 : (size_t)sp * m_max * tsize));
     col = o; o = align16(o + (size_t)m_max * tsize);
     w = o; o = align16(o + (size_t)m_max * tsize);
     xs = o; o = align16(o + (size_t)m_max * dd * tsize);
@@ -951,15 +954,44 @@ struct PlanRegLds {
 // MODE 1: the first pass and support-update passes; MODE 2: residual passes
 // (anything).  X stays in registers until pass 4 only where pass 4 needs it:
 // residuals, and FP64, whose Y update keeps the reference's Y + (X - Z).
-template <typename T, int CM, int DIM, int NC, int MODE>
+//
+// PF (lean FP32 passes): the next object's Z and Y are copied from HBM
+// straight into LDS (global_load_lds, no registers) while the current object
+// computes, so the workgroup's next round trip overlaps its own passes 3 and
+// 4.  The copy is issued once pass 2 has read t (which lives in the Z staging
+// array); the barriers after it wait on LDS only, so the copy stays in flight
+// until the next object's start waits for it.
+constexpr int PF_STAGE = 4608;  // staging floats per array: s * m_max and sp * m_max
+
+// workgroup barrier that waits for this wave's LDS operations only (not for
+// its outstanding vector-memory loads and stores); the empty asm statements
+// keep the compiler from moving memory accesses across it
+__device__ __forceinline__ void lds_sync() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0); vmcnt, expcnt unconstrained (gfx9 encoding)
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+// workgroup barrier after all of this wave's memory operations completed
+__device__ __forceinline__ void full_sync() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt(0);
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
+template <typename T, int CM, int DIM, int NC, int MODE, bool PF = false>
 __global__ __launch_bounds__(NT)
 __attribute__((amdgpu_waves_per_eu(
-    MODE == 0 || (MODE == 1 && D2MI_PLANREG_MID_CAP && !std::is_same<T, double>::value)
+    PF ? 4  // the staging arrays (36 KB) allow 4 workgroups per CU
+    : MODE == 0 || (MODE == 1 && D2MI_PLANREG_MID_CAP && !std::is_same<T, double>::value)
         ? D2MI_PLANREG_WAVES_PER_EU(NC)
         : (MODE == 2 && !std::is_same<T, double>::value ? D2MI_PLANREG_RESID_WAVES : 1))))
 void k_badmm_plan_reg(PlanArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr bool F64 = std::is_same<T, double>::value;
+  static_assert(!PF || (MODE == 0 && !F64 && CM != COST_TABLE), "prefetch: lean FP32 passes");
+  __shared__ __align__(16) T stZ[PF ? PF_STAGE : 1], stY[PF ? PF_STAGE : 1];
   constexpr int U = PLAN_UNROLL;
   typedef typename Vec2<T>::type T2;
   const int s = a.s, d = DIM > 0 ? DIM : a.d, sp = a.sp, tid = threadIdx.x;
@@ -967,8 +999,9 @@ void k_badmm_plan_reg(PlanArgs<T> a) {
   const bool upd = MODE >= 1 && a.upd, first = MODE >= 1 && a.first;
   const bool resid = MODE == 2 && a.resid;
   const bool keepx = MODE >= 1 && (F64 || resid);  // x held until pass 4
-  const PlanRegLds L(sizeof(T), sp, a.m_max, s, d, a.R);
-  T *A = (T *)(smem + L.A), *csl = (T *)(smem + L.col);
+  const PlanRegLds L(sizeof(T), sp, a.m_max, s, d, a.R, PF);
+  T *A = PF ? stZ : (T *)(smem + L.A);
+  T *csl = (T *)(smem + L.col);
   T *wl = (T *)(smem + L.w), *xs = (T *)(smem + L.xs);
   T *cs = (T *)(smem + L.cs), *cwl = (T *)(smem + L.cw), *g = (T *)(smem + L.g);
   T *rp = (T *)(smem + L.rp);
@@ -987,6 +1020,24 @@ void k_badmm_plan_reg(PlanArgs<T> a) {
   double r_obj = 0.0, r_pr = 0.0, r_dr = 0.0;
   for (int q = tid; q < a.R; q += NT) acc[q] = 0.0;
   int cur = -1;
+  // PF: object pq's Z and Y blocks (s m contiguous floats each) into stZ, stY,
+  // 64 elements per wave instruction (the LDS address is wave-uniform)
+  auto prefetch = [&](int pq) {
+    if constexpr (PF) {
+      const int oq = const_load(a.perm + pq);
+      const int mq = const_load(a.p_str + oq);
+      const size_t bq = (size_t)s * const_load(a.cum + oq);
+      const int n = s * mq;
+      for (int i = wv; i * 64 < n; i += NT / 64) {
+        const int e = i * 64 + lane;
+        if (e < n) {
+          __builtin_amdgcn_global_load_lds(a.Z + bq + e, stZ + i * 64, 4, 0, 0);
+          __builtin_amdgcn_global_load_lds(a.Y + bq + e, stY + i * 64, 4, 0, 0);
+        }
+      }
+    }
+  };
+  if (PF && p0 < p1) prefetch(p0);
 
   for (int p = p0; p < p1; ++p) {
     // per-object copies of the thread's row and first column: without this
@@ -1005,15 +1056,29 @@ void k_badmm_plan_reg(PlanArgs<T> a) {
     const int voff = (ke + jb * s) * (int)sizeof(T), cstride = cpi * s * (int)sizeof(T);
     // requests first: plan elements, weights, supports or bin ids
     T zr[NC], yr[NC], xr[NC], cr[NC];
+    if constexpr (PF) {
+      full_sync();  // every wave's share of this object's copy has landed
 #pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      const int j = jb + c * cpi;
-      zr[c] = (T)0;
-      yr[c] = (T)0;
-      if (j < m) {
-        if (!reinit) zr[c] = buf_load<T>(rz, voff, c * cstride);
-        if (!first) yr[c] = buf_load<T>(ry, voff, c * cstride);
-        if constexpr (CM == COST_TABLE) cr[c] = a.ctab[base + ke + j * s];
+      for (int c = 0; c < NC; ++c) {
+        const int j = jb + c * cpi;
+        zr[c] = (T)0;
+        yr[c] = (T)0;
+        if (j < m) {
+          zr[c] = stZ[ke + j * s];
+          yr[c] = stY[ke + j * s];
+        }
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        const int j = jb + c * cpi;
+        zr[c] = (T)0;
+        yr[c] = (T)0;
+        if (j < m) {
+          if (!reinit) zr[c] = buf_load<T>(rz, voff, c * cstride);
+          if (!first) yr[c] = buf_load<T>(ry, voff, c * cstride);
+          if constexpr (CM == COST_TABLE) cr[c] = a.ctab[base + ke + j * s];
+        }
       }
     }
     T wpre[PRE_W], xpre[PRE_X];
@@ -1120,6 +1185,8 @@ void k_badmm_plan_reg(PlanArgs<T> a) {
       if (q4 == 0) csl[j] = F64 ? part : wl[j] / part;
     }
     __syncthreads();
+    // t (in stZ) is read: the next object's copy can start
+    if (PF && p + 1 < p1) prefetch(p + 1);
     // pass 3 -- step 1c: x = t / colsum * w (:159); step 2a: z' = x exp(Y) +
     // 1e-9 (:166); partial row sums (_drnorm, :169) and step 5 partials
     {
@@ -1154,7 +1221,8 @@ void k_badmm_plan_reg(PlanArgs<T> a) {
         }
       }
     }
-    __syncthreads();
+    if constexpr (PF) lds_sync();
+    else __syncthreads();
     // step 4: normalised row sums summed per label (:180-185)
     if (wv == 0) {
       T r = 0;
@@ -1173,7 +1241,8 @@ void k_badmm_plan_reg(PlanArgs<T> a) {
         }
       }
     }
-    __syncthreads();
+    if constexpr (PF) lds_sync();
+    else __syncthreads();
     // pass 4 -- step 2c + 3: z = z' / rowsum * c_w (:169-170); Y += X - Z (:175)
     {
       const T gk = g[ke];
@@ -2238,7 +2307,7 @@ int occupancy_of(F kernel, size_t lds, int threads = NT) {
 // (plan in LDS, s <= 64), k_badmm_plan (any s) or its HBM-scratch form (GM);
 // LEAN instances serve passes after the first with no support update,
 // residual or kept X
-enum PlanKind { PK_LEGACY, PK_GM, PK_64, PK_REG8, PK_REG16, PK_TALL };
+enum PlanKind { PK_LEGACY, PK_GM, PK_64, PK_REG8, PK_REG16, PK_TALL, PK_REG8P, PK_REG16P };
 
 const char *plan_kind_name(PlanKind k) {
   switch (k) {
@@ -2247,11 +2316,19 @@ const char *plan_kind_name(PlanKind k) {
     case PK_REG8: return "reg8";
     case PK_REG16: return "reg16";
     case PK_TALL: return "tall";
+    case PK_REG8P: return "reg8p";
+    case PK_REG16P: return "reg16p";
     default: return "general";
   }
 }
 
 inline bool is_tall(PlanKind k) { return k == PK_TALL; }
+// register forms; the P kinds run their lean passes with the next object's
+// plan copied into LDS ahead of time (k_badmm_plan_reg<..., PF = true>)
+inline bool is_reg(PlanKind k) {
+  return k == PK_REG8 || k == PK_REG16 || k == PK_REG8P || k == PK_REG16P;
+}
+inline bool is_pf(PlanKind k) { return k == PK_REG8P || k == PK_REG16P; }
 // workgroup size of a plan kernel: one wave per 64 rows for the tall form
 inline int plan_threads(PlanKind k, int s) { return is_tall(k) ? (s + 63) / 64 * 64 : NT; }
 
@@ -2266,6 +2343,7 @@ struct PlanKernels {
   // lean instances exist for FP32 only (FP64 passes always keep X), and FP64
   // register forms for up to 8 columns per thread
   static constexpr bool F32 = !std::is_same<T, double>::value;
+  static constexpr bool PFOK = F32 && CM != COST_TABLE;  // prefetch form instances
   // register-form mode: 0 lean, 1 first / support-update passes, 2 residual
   template <bool LEAN>
   static int reg_mode(const PlanArgs<T> &pa) {
@@ -2315,6 +2393,9 @@ struct PlanKernels {
   static int occ(size_t lds, PlanKind k, int mode, int s, int mc) {
     switch (k) {
       case PK_TALL: return tall_occ(mc, LEAN ? 0 : mode, plan_threads(k, s));
+      case PK_REG16P:
+        if constexpr (PFOK && LEAN) return occupancy_of(k_badmm_plan_reg<T, CM, DIM, 16, 0, true>, lds);
+        [[fallthrough]];
       case PK_REG16:
         if constexpr (F32) {
           if constexpr (LEAN) return occupancy_of(k_badmm_plan_reg<T, CM, DIM, 16, 0>, lds);
@@ -2322,6 +2403,9 @@ struct PlanKernels {
                            : occupancy_of(k_badmm_plan_reg<T, CM, DIM, 16, 1>, lds);
         }
         return 0;
+      case PK_REG8P:
+        if constexpr (PFOK && LEAN) return occupancy_of(k_badmm_plan_reg<T, CM, DIM, 8, 0, true>, lds);
+        [[fallthrough]];
       case PK_REG8:
         if constexpr (LEAN) return occupancy_of(k_badmm_plan_reg<T, CM, DIM, 8, 0>, lds);
         return mode == 2 ? occupancy_of(k_badmm_plan_reg<T, CM, DIM, 8, 2>, lds)
@@ -2336,6 +2420,12 @@ struct PlanKernels {
     const int mode = reg_mode<LEAN>(pa);
     switch (k) {
       case PK_TALL: tall_run(G, S, pa, mode); break;
+      case PK_REG16P:
+        if constexpr (PFOK && LEAN) {
+          k_badmm_plan_reg<T, CM, DIM, 16, 0, true><<<G, NT, lds, S>>>(pa);
+          break;
+        }
+        [[fallthrough]];
       case PK_REG16:
         if constexpr (F32) {
           if constexpr (LEAN) k_badmm_plan_reg<T, CM, DIM, 16, 0><<<G, NT, lds, S>>>(pa);
@@ -2343,6 +2433,12 @@ struct PlanKernels {
           else k_badmm_plan_reg<T, CM, DIM, 16, 1><<<G, NT, lds, S>>>(pa);
         }
         break;
+      case PK_REG8P:
+        if constexpr (PFOK && LEAN) {
+          k_badmm_plan_reg<T, CM, DIM, 8, 0, true><<<G, NT, lds, S>>>(pa);
+          break;
+        }
+        [[fallthrough]];
       case PK_REG8:
         if constexpr (LEAN) k_badmm_plan_reg<T, CM, DIM, 8, 0><<<G, NT, lds, S>>>(pa);
         else if (mode == 2) k_badmm_plan_reg<T, CM, DIM, 8, 2><<<G, NT, lds, S>>>(pa);
@@ -2717,16 +2813,22 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
     const int spl = (s % 2 == 0) ? s + 1 : s;  // odd column stride: conflict-free sums
     auto set_kind = [&](Bucket &B, PlanKind k) {
       B.kind = k;
-      B.k64 = k == PK_64 || k == PK_REG8 || k == PK_REG16;
+      B.k64 = k == PK_64 || is_reg(k);
       B.gm = k == PK_GM;
       B.sp = B.k64 ? sp64 : spl;
     };
-    auto fits = [&](int m_lim, PlanKind k) -> bool {
+    std::function<bool(int, PlanKind)> fits = [&](int m_lim, PlanKind k) -> bool {
       const int ncols = s <= 64 ? (m_lim + NT / s - 1) / (NT / s) : 1 << 30;
       switch (k) {
         case PK_TALL:  // class limits are multiples of 8 (the kernel's column slots)
           return CM == COST_SPARSE && s > 64 && s <= TALL_NT && m_lim % 8 == 0 &&
                  m_lim <= (F64 ? TALL_MC64 : TALL_MC32);
+        case PK_REG16P:
+        case PK_REG8P:  // also: the staging arrays hold t (sp x m) and the block (s x m)
+          if (F64 || CM == COST_TABLE || s < 7 || (size_t)sp64 * m_lim > PF_STAGE ||
+              (size_t)s * m_lim > PF_STAGE)
+            return false;
+          return fits(m_lim, k == PK_REG16P ? PK_REG16 : PK_REG8);
         case PK_REG16:
           if (F64) return false;
           [[fallthrough]];
@@ -2759,6 +2861,11 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
           k = PK_64;
           if (!no_reg && fits(B.m_max, PK_REG8)) k = PK_REG8;
           else if (!no_reg && fits(B.m_max, PK_REG16)) k = PK_REG16;
+          // D2MI_PLAN_PREFETCH=1: the register forms' prefetching variants
+          if (getenv_flag("D2MI_PLAN_PREFETCH")) {
+            if (k == PK_REG8 && fits(B.m_max, PK_REG8P)) k = PK_REG8P;
+            if (k == PK_REG16 && fits(B.m_max, PK_REG16P)) k = PK_REG16P;
+          }
         } else if (fits(B.m_max, PK_LEGACY)) {
           k = PK_LEGACY;
         }
@@ -2773,10 +2880,12 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
       set_kind(B, k);
       any_gm = any_gm || B.gm;
     }
-    auto lds_of = [&](const Bucket &B, int keepx, int R) -> size_t {
+    // dynamic LDS of bucket B's kernel for a pass; lean: the prefetch form's
+    // lean passes keep t in their static staging array
+    auto lds_of = [&](const Bucket &B, int keepx, int R, bool lean = false) -> size_t {
       if (is_tall(B.kind)) return 0;  // static LDS only
-      if (B.kind == PK_REG8 || B.kind == PK_REG16)
-        return PlanRegLds(sizeof(T), B.sp, B.m_max, s, d, R).total;
+      if (is_reg(B.kind))
+        return PlanRegLds(sizeof(T), B.sp, B.m_max, s, d, R, lean && is_pf(B.kind)).total;
       return B.k64 ? Plan64Lds(sizeof(T), B.sp, B.m_max, s, d, keepx, R).total
                    : PlanLds(sizeof(T), B.sp, B.m_max, s, d, keepx, R, B.gm).total;
     };
@@ -2850,7 +2959,8 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
         // than 3% faster on the plan passes, since more workgroups also cost
         // the per-pass reduction (more records) that these timings leave out
         std::vector<PlanTuneVal> cands{PlanTuneVal{(int)B.kind, 0}};
-        const PlanKind kinds[] = {PK_TALL, PK_REG8, PK_REG16, PK_64, PK_LEGACY, PK_GM};
+        const PlanKind kinds[] = {PK_TALL, PK_REG8, PK_REG16, PK_REG8P, PK_REG16P, PK_64,
+                                  PK_LEGACY, PK_GM};
         bool any_fast = false;
         for (PlanKind k : kinds)
           if (k != PK_GM && fits(B.m_max, k)) any_fast = true;
@@ -2873,7 +2983,7 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
           B.wg_mode = wm;
           B.rec_off = 0;
           const bool lean = !F64 && !B.gm && !no_lean;
-          const size_t lds = lds_of(B, F64 ? 1 : 0, s);
+          const size_t lds = lds_of(B, F64 ? 1 : 0, s, lean);
           if (wm == 1 && occupancy(B, lds, lean, false) <= 1) continue;
           int G, chunk;
           geometry(B, lds, &G, &chunk, lean, false);
@@ -2921,15 +3031,18 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
       }
       int chunk_tmp = 0;
       B.Gmax = 0;
-      if (B.count > 0)
-        geometry(B, lds_of(B, 0, s), &B.Gmax, &chunk_tmp, !F64 && !B.gm && !no_lean, false);
+      if (B.count > 0) {
+        const bool lean0 = !F64 && !B.gm && !no_lean;
+        geometry(B, lds_of(B, 0, s, lean0), &B.Gmax, &chunk_tmp, lean0, false);
+      }
       B.rec_off = rec_total;
       rec_total += B.Gmax + K;
       part_total += B.Gmax;
       if (B.count > 0) {
         char f[64];
+        const bool lean0 = !F64 && !B.gm && !no_lean;
         snprintf(f, sizeof f, "%sm<=%d:%s/%d(%d)", forms.empty() ? "" : " ", B.m_max,
-                 plan_kind_name(B.kind), per_cu_of(B, lds_of(B, 0, s), !F64 && !B.gm && !no_lean, false),
+                 plan_kind_name(B.kind), per_cu_of(B, lds_of(B, 0, s, lean0), lean0, false),
                  B.count);
         forms += f;
       }
@@ -2954,8 +3067,8 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
           fa.rec_off[b] = B.rec_off;
           fa.chunk[b] = 1;
           if (B.count == 0) continue;
-          const size_t lds = lds_of(B, keepx, R);
           const bool lean = !F64 && !B.gm && !no_lean && iter > 0 && !upd && !resid && !keepx;
+          const size_t lds = lds_of(B, keepx, R, lean);
           int G, chunk;
           geometry(B, lds, &G, &chunk, lean, resid);
           if (G > B.Gmax) {  // records are sized for Gmax workgroups

@@ -194,3 +194,21 @@ def test_rho_from_line_sums_histograms(engine, which):
     co = c0.copy()
     ro = O.badmm(ph, lab, np.ones(ph.size, np.int8), co, np.zeros(c0.s * ph.col), opts)
     assert abs(sa["rho"] - ro["rho"]) <= 1e-12 * ro["rho"]
+
+
+@pytest.mark.parametrize("n,seed", [(4000, 3), (600, 9)])
+def test_prefetch_form_bit_identical(engine, n, seed):
+    """The register form's prefetching variant (the next object's plan copied
+    into LDS while the current one computes) does the same arithmetic in the
+    same order: FP32 updates agree bit for bit with the plain register form,
+    across size classes (reg8p for m <= 48, reg16p above)."""
+    say(f"prefetch form, grid strokes N={n}")
+    data, ph, c0, lab = strokes(n, seed=seed)
+    opts = BADMMOptions(10, 2.0, 10)
+    sa, ca, za = update(engine, data, lab, c0, opts, 32, {"D2MI_PLAN_NO_BUCKETS": "0"})
+    sb, cb, zb = update(engine, data, lab, c0, opts, 32, {"D2MI_PLAN_PREFETCH": "1"})
+    assert "p" in sb["plan_forms"].split(":")[1].split("/")[0], sb["plan_forms"]
+    np.testing.assert_array_equal(ca.p_w, cb.p_w)
+    np.testing.assert_array_equal(ca.p_supp, cb.p_supp)
+    np.testing.assert_array_equal(za, zb)
+    assert sa["obj"] == sb["obj"]
