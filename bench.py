@@ -41,6 +41,13 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+# The assignment server keeps one HIP stream per launch group in flight, and
+# HIP maps streams onto GPU_MAX_HW_QUEUES hardware queues (4 by default):
+# groups whose streams share a queue run one after the other.  16 gives each
+# of the server's streams a queue of its own.  Read when HIP initialises, so
+# it is set before anything touches the GPU; an explicit setting wins.
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+
 METRIC = "B-ADMM plan updates/s at obj rel err ≤1e-5 vs FP64; assign p50/p99 ms"
 HBM_PEAK_GBPS = 8000.0
 
@@ -180,6 +187,7 @@ def assign_leg(eng, args):
         "train_seconds": train_s,
         "queries": int(qph.size), "qps_offered": args.assign_qps,
         "max_batch": args.assign_batch, "streams": args.assign_streams,
+        "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES"),
         "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
         "qps_achieved": qph.size / el, "burst_qps": burst,
         "lps_per_query": st["lps"] / max(st["completed"], 1), "groups": st["groups"],
@@ -466,8 +474,8 @@ def main():
     ap.add_argument("--multiphase-n", type=int, default=100_000)
     ap.add_argument("--assign-n", type=int, default=3000)
     ap.add_argument("--assign-qps", type=float, default=1000.0)
-    ap.add_argument("--assign-batch", type=int, default=48)
-    ap.add_argument("--assign-streams", type=int, default=6)
+    ap.add_argument("--assign-batch", type=int, default=16)
+    ap.add_argument("--assign-streams", type=int, default=12)
     ap.add_argument("--assign-plain-n", type=int, default=300)
     ap.add_argument("--assign-check", type=int, default=100)
     ap.add_argument("--assign-train", type=int, default=4000)

@@ -244,6 +244,33 @@ __global__ void k_srv_final(int R, int K, const unsigned char *__restrict__ mask
   atomicAdd(nlp, (unsigned long long)cnt);
 }
 
+// The jobs (query, centroid) a round solves, compacted in job order into
+// jobs[0 .. *count): each then gets a workgroup of its own instead of the
+// launch striding over every (query, centroid) pair, where workgroups that
+// drew two live jobs solved them one after the other.  One workgroup.
+constexpr int CP_T = 1024;
+__global__ __launch_bounds__(CP_T) void k_srv_compact(const unsigned char *__restrict__ mask,
+                                                      int n, int *__restrict__ jobs,
+                                                      int *__restrict__ count) {
+  __shared__ int sums[CP_T];
+  const int t = threadIdx.x, per = (n + CP_T - 1) / CP_T;
+  const int lo = min(n, t * per), hi = min(n, lo + per);
+  int c = 0;
+  for (int i = lo; i < hi; ++i) c += mask[i] != 0;
+  sums[t] = c;
+  __syncthreads();
+  for (int off = 1; off < CP_T; off <<= 1) {  // inclusive scan (Hillis-Steele)
+    const int v = t >= off ? sums[t - off] : 0;
+    __syncthreads();
+    sums[t] += v;
+    __syncthreads();
+  }
+  int o = sums[t] - c;
+  for (int i = lo; i < hi; ++i)
+    if (mask[i]) jobs[o++] = i;
+  if (t == CP_T - 1) *count = sums[t];
+}
+
 __global__ void k_srv_zero_status(int *st, unsigned long long *nlp) {
   if (threadIdx.x == 0) { st[0] = 0; *nlp = 0; }
 }
@@ -347,7 +374,7 @@ bool slot_serves(const d2mi_server *S, int slot, int cls) {
 // device bytes of a group of R queries with `col` support points in total
 struct GroupLayout {
   size_t p_str, cum, w, supp, in_end;                            // inputs (one H2D copy)
-  size_t lb, vals, mask, mask2, label, dist, nlp, status, out_end;  // outputs from `label`
+  size_t lb, vals, mask, mask2, jobs, njob, label, dist, nlp, status, out_end;  // outputs from `label`
   size_t total;
   GroupLayout(int R, size_t col, int d, int K) {
     size_t o = 0;
@@ -360,6 +387,8 @@ struct GroupLayout {
     vals = o; o = a256(o + sizeof(double) * (size_t)R * K);
     mask = o; o = a256(o + (size_t)R * K);
     mask2 = o; o = a256(o + (size_t)R * K);
+    jobs = o; o = a256(o + sizeof(int) * (size_t)R * K);
+    njob = o; o = a256(o + sizeof(int) * 2);
     label = o; o = a256(o + sizeof(int) * R);
     dist = o; o = a256(o + sizeof(double) * R);
     nlp = o; o = a256(o + sizeof(unsigned long long));
@@ -401,6 +430,11 @@ int launch_lp_round(d2mi_server *S, SrvGroup &G, const GroupLayout &L, char *bas
   a.njobs = R * S->K;
   a.implicit_K = S->K;
   a.mask = (const unsigned char *)(base + (round == 1 ? L.mask : L.mask2));
+  int *jobs = (int *)(base + L.jobs), *njob = (int *)(base + L.njob) + (round - 1);
+  k_srv_compact<<<1, CP_T, 0, G.stream>>>(a.mask, a.njobs, jobs, njob);
+  D2MI_HIP(hipGetLastError());
+  a.job_map = jobs;
+  a.njobs_dev = njob;
   a.out = (double *)(base + L.vals);
   return d2mi_launch_transport(ctx, a);
 }

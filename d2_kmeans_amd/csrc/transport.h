@@ -23,6 +23,8 @@ struct LpArgs {
   const int *job_map;          // with implicit_K: launch job t is global job job_map[t]
                                // (mask, out and the pair are those of the global job)
   const unsigned char *mask;   // optional: jobs with mask[job] == 0 are skipped
+  const int *njobs_dev;        // optional: the launch-job count, read on the device
+                               // (at most njobs; with job_map, a compacted job list)
   int b_is_table;
   int cost_mode;
   int d;

@@ -25,7 +25,8 @@ __global__ __launch_bounds__(1024) void k_transport(LpArgs a) {
   double *A = (double *)(lds + L.A), *B = (double *)(lds + L.B);
   double *wa = (double *)(lds + L.wa), *wb = (double *)(lds + L.wb);
   int *ids = (int *)(lds + L.ids);
-  for (int t = blockIdx.x; t < a.njobs; t += gridDim.x) {
+  const int njobs = a.njobs_dev ? min(*a.njobs_dev, a.njobs) : a.njobs;
+  for (int t = blockIdx.x; t < njobs; t += gridDim.x) {
     const int job = a.job_map ? a.job_map[t] : t;
     if (a.mask && a.mask[job] == 0) continue;
     // row side: centroid `ka` of table A
