@@ -45,8 +45,13 @@ sys.path.insert(0, ROOT)
 # HIP maps streams onto GPU_MAX_HW_QUEUES hardware queues (4 by default):
 # groups whose streams share a queue run one after the other.  16 gives each
 # of the server's streams a queue of its own.  Read when HIP initialises, so
-# it is set before anything touches the GPU; an explicit setting wins.
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+# it is set before anything touches the GPU (--hw-queues 0 keeps the
+# environment's value).
+HW_QUEUES = 16
+if "--hw-queues" in sys.argv:
+    HW_QUEUES = int(sys.argv[sys.argv.index("--hw-queues") + 1])
+if HW_QUEUES > 0:
+    os.environ["GPU_MAX_HW_QUEUES"] = str(HW_QUEUES)
 
 METRIC = "B-ADMM plan updates/s at obj rel err ≤1e-5 vs FP64; assign p50/p99 ms"
 HBM_PEAK_GBPS = 8000.0
@@ -474,8 +479,12 @@ def main():
     ap.add_argument("--multiphase-n", type=int, default=100_000)
     ap.add_argument("--assign-n", type=int, default=3000)
     ap.add_argument("--assign-qps", type=float, default=1000.0)
-    ap.add_argument("--assign-batch", type=int, default=16)
-    ap.add_argument("--assign-streams", type=int, default=12)
+    ap.add_argument("--assign-batch", type=int, default=0,
+                    help="queries per launch group (0: 16 with >= 13 hardware queues, else 48)")
+    ap.add_argument("--assign-streams", type=int, default=0,
+                    help="server streams (0: 12 with >= 13 hardware queues, else 6)")
+    ap.add_argument("--hw-queues", type=int, default=HW_QUEUES,
+                    help="GPU_MAX_HW_QUEUES for this process (0: keep the environment's)")
     ap.add_argument("--assign-plain-n", type=int, default=300)
     ap.add_argument("--assign-check", type=int, default=100)
     ap.add_argument("--assign-train", type=int, default=4000)
@@ -484,6 +493,11 @@ def main():
                     help="also run one FP32->FP64 plan copy kernel (reads 4 B, writes 8 B per "
                          "plan element) to scale rocprof byte counters")
     args = ap.parse_args()
+    hwq = int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))
+    if args.assign_batch <= 0:
+        args.assign_batch = 16 if hwq >= 13 else 48
+    if args.assign_streams <= 0:
+        args.assign_streams = 12 if hwq >= 13 else 6
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(see the sharding design); run with --gpus 1", file=sys.stderr)
