@@ -140,21 +140,78 @@ __device__ __forceinline__ void split_cell(int e, int n, float inv_n, int &k, in
   j = q;
 }
 
-__device__ __forceinline__ double wave_min_d(double v) {
-  for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, WAVE));
-  return v;
+// Cross-lane moves for the selection reductions (min, max, argmin with an
+// index tie-break, top-2): DPP, plain VALU moves, where __shfl_xor is an LDS
+// round trip (ds_bpermute) per step.  Within each 16-lane row, rotations by
+// 1, 2, 4, 8 leave every lane with the row's result; row_bcast:15 into rows 1
+// and 3, then row_bcast:31 into rows 2 and 3, carry it to lane 63 (rows the
+// row mask leaves out read `id`, the reduction's identity), which is read back
+// as a scalar.  Selections are exact, so the order does not matter: results
+// equal those of any other order.  All 64 lanes must be active.
+enum { DPP_ROR1 = 0x121, DPP_ROR2 = 0x122, DPP_ROR4 = 0x124, DPP_ROR8 = 0x128,
+       DPP_BC15 = 0x142, DPP_BC31 = 0x143 };
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ int dpp_i(int v, int id = 0) {
+  return __builtin_amdgcn_update_dpp(id, v, CTRL, ROWS, 0xf, false);
 }
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ float dpp_f(float v, float id = 0.f) {
+  return __builtin_bit_cast(float, dpp_i<CTRL, ROWS>(__builtin_bit_cast(int, v),
+                                                     __builtin_bit_cast(int, id)));
+}
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ double dpp_d(double v, double id = 0.0) {
+  typedef int i2 __attribute__((ext_vector_type(2)));
+  const i2 u = __builtin_bit_cast(i2, v), w = __builtin_bit_cast(i2, id);
+  i2 r;
+  r.x = dpp_i<CTRL, ROWS>(u.x, w.x);
+  r.y = dpp_i<CTRL, ROWS>(u.y, w.y);
+  return __builtin_bit_cast(double, r);
+}
+__device__ __forceinline__ int lane63_i(int v) { return __builtin_amdgcn_readlane(v, 63); }
+__device__ __forceinline__ float lane63_f(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 63));
+}
+__device__ __forceinline__ double lane63_d(double v) {
+  typedef int i2 __attribute__((ext_vector_type(2)));
+  i2 u = __builtin_bit_cast(i2, v);
+  u.x = __builtin_amdgcn_readlane(u.x, 63);
+  u.y = __builtin_amdgcn_readlane(u.y, 63);
+  return __builtin_bit_cast(double, u);
+}
+
+__device__ __forceinline__ double wave_min_d(double v) {
+  v = fmin(v, dpp_d<DPP_ROR1>(v));
+  v = fmin(v, dpp_d<DPP_ROR2>(v));
+  v = fmin(v, dpp_d<DPP_ROR4>(v));
+  v = fmin(v, dpp_d<DPP_ROR8>(v));
+  v = fmin(v, dpp_d<DPP_BC15, 0xa>(v, DBL_MAX));
+  v = fmin(v, dpp_d<DPP_BC31, 0xc>(v, DBL_MAX));
+  return lane63_d(v);
+}
+// a sum is not a selection: this one keeps its butterfly order (values feed
+// the supplies, and a different rounding could change ties in the start)
 __device__ __forceinline__ double wave_sum_d(double v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
   return v;
 }
 __device__ __forceinline__ int wave_min_i(int v) {
-  for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, WAVE));
-  return v;
+  v = min(v, dpp_i<DPP_ROR1>(v));
+  v = min(v, dpp_i<DPP_ROR2>(v));
+  v = min(v, dpp_i<DPP_ROR4>(v));
+  v = min(v, dpp_i<DPP_ROR8>(v));
+  v = min(v, dpp_i<DPP_BC15, 0xa>(v, INT_MAX));
+  v = min(v, dpp_i<DPP_BC31, 0xc>(v, INT_MAX));
+  return lane63_i(v);
 }
 __device__ __forceinline__ double wave_max_d(double v) {
-  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, WAVE));
-  return v;
+  v = fmax(v, dpp_d<DPP_ROR1>(v));
+  v = fmax(v, dpp_d<DPP_ROR2>(v));
+  v = fmax(v, dpp_d<DPP_ROR4>(v));
+  v = fmax(v, dpp_d<DPP_ROR8>(v));
+  v = fmax(v, dpp_d<DPP_BC15, 0xa>(v, -DBL_MAX));
+  v = fmax(v, dpp_d<DPP_BC31, 0xc>(v, -DBL_MAX));
+  return lane63_d(v);
 }
 
 // Control block in LDS: reduction slots (two banks, alternated per use so a
@@ -253,17 +310,31 @@ __device__ __forceinline__ Top2 top2_insert(Top2 t, float c, int i) {
   return r;
 }
 
-// wave-wide merge, every lane gets the result
+// wave-wide merge, every lane gets the result (DPP as wave_min_d: each lane's
+// pair is merged exactly once, and the two smallest (value, index) pairs of
+// distinct cells do not depend on the merge order)
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ Top2 top2_merge_dpp(Top2 t) {
+  Top2 o;
+  o.v1 = dpp_f<CTRL, ROWS>(t.v1, FLT_MAX);
+  o.v2 = dpp_f<CTRL, ROWS>(t.v2, FLT_MAX);
+  o.a1 = dpp_i<CTRL, ROWS>(t.a1, INT_MAX);
+  o.a2 = dpp_i<CTRL, ROWS>(t.a2, INT_MAX);
+  return top2_insert(top2_insert(t, o.v1, o.a1), o.v2, o.a2);
+}
 __device__ __forceinline__ Top2 wave_top2(Top2 t) {
-  for (int off = 32; off > 0; off >>= 1) {
-    Top2 o;
-    o.v1 = __shfl_xor(t.v1, off, WAVE);
-    o.v2 = __shfl_xor(t.v2, off, WAVE);
-    o.a1 = __shfl_xor(t.a1, off, WAVE);
-    o.a2 = __shfl_xor(t.a2, off, WAVE);
-    t = top2_insert(top2_insert(t, o.v1, o.a1), o.v2, o.a2);
-  }
-  return t;
+  t = top2_merge_dpp<DPP_ROR1>(t);
+  t = top2_merge_dpp<DPP_ROR2>(t);
+  t = top2_merge_dpp<DPP_ROR4>(t);
+  t = top2_merge_dpp<DPP_ROR8>(t);
+  t = top2_merge_dpp<DPP_BC15, 0xa>(t);
+  t = top2_merge_dpp<DPP_BC31, 0xc>(t);
+  Top2 r;
+  r.v1 = lane63_f(t.v1);
+  r.v2 = lane63_f(t.v2);
+  r.a1 = lane63_i(t.a1);
+  r.a2 = lane63_i(t.a2);
+  return r;
 }
 
 // Vogel's approximation, run by wave 0 alone (no workgroup barrier): each of
@@ -356,11 +427,20 @@ __device__ void vogel_start(const Problem &P, unsigned char *lds, const Layout &
       bp = better ? pen : bp;
       bl = better ? v : bl;
     }
-    for (int off = 32; off > 0; off >>= 1) {
-      const float op = __shfl_xor(bp, off, WAVE);
-      const int ol = __shfl_xor(bl, off, WAVE);
-      if (op > bp || (op == bp && ol < bl)) { bp = op; bl = ol; }
-    }
+    // wave argmax (largest penalty, lowest line on ties), DPP as wave_min_d
+    auto pen_merge = [&](float op, int ol) {
+      const bool better = op > bp || (op == bp && ol < bl);
+      bp = better ? op : bp;
+      bl = better ? ol : bl;
+    };
+    pen_merge(dpp_f<DPP_ROR1>(bp), dpp_i<DPP_ROR1>(bl));
+    pen_merge(dpp_f<DPP_ROR2>(bp), dpp_i<DPP_ROR2>(bl));
+    pen_merge(dpp_f<DPP_ROR4>(bp), dpp_i<DPP_ROR4>(bl));
+    pen_merge(dpp_f<DPP_ROR8>(bp), dpp_i<DPP_ROR8>(bl));
+    pen_merge(dpp_f<DPP_BC15, 0xa>(bp, -FLT_MAX), dpp_i<DPP_BC15, 0xa>(bl, INT_MAX));
+    pen_merge(dpp_f<DPP_BC31, 0xc>(bp, -FLT_MAX), dpp_i<DPP_BC31, 0xc>(bl, INT_MAX));
+    bp = lane63_f(bp);
+    bl = lane63_i(bl);
     const int is = bl < n ? bl : ma1[bl];
     const int js = bl < n ? ma1[bl] : bl - n;
     const double ra = rem[is], rb = rem[n + js];
