@@ -40,7 +40,12 @@ __global__ void k_row_norms(const float *__restrict__ x, int P, int d, float *__
   nx[p] = s;
 }
 
-template <bool BF16>
+// VEC (d % 4 == 0, so rows start 16-byte aligned): each thread fetches its
+// share of the next 32-wide slice of the shared dimension as float4 into
+// registers right after the barrier, so the loads are in flight while the
+// matrix cores work on the current slice; otherwise the slice is fetched
+// element by element before each step.
+template <bool BF16, bool VEC>
 __global__ __launch_bounds__(NTH) void k_pdist2_mfma(const float *__restrict__ x,
                                                      const float *__restrict__ y, int P, int Q,
                                                      int d, const float *__restrict__ nx,
@@ -68,19 +73,44 @@ __global__ __launch_bounds__(NTH) void k_pdist2_mfma(const float *__restrict__ x
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[a][c][r] = 0.f;
 
-  for (int k0 = 0; k0 < d; k0 += KC) {
-    // stage: each thread loads 16 floats of x and 16 of y (rows of 32 k)
-    for (int e = tid; e < TM * KC; e += NTH) {
-      const int r = e / KC, k = e - (e / KC) * KC;
-      const int p = p0 + r, kk = k0 + k;
-      Xs[r * LDA + k] = (p < P && kk < d) ? x[(size_t)p * d + kk] : 0.f;
+  // VEC: 4 float4 of x and 4 of y per thread per slice; group g = tid + i NTH
+  // is row g / 8, columns 4 (g % 8) .. +3 of the slice
+  constexpr int NV = TM * KC / 4 / NTH;
+  float4 px[VEC ? NV : 1], py[VEC ? NV : 1];
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      const int g = tid + i * NTH, r = g >> 3, kk = k0 + (g & 7) * 4;
+      const int p = p0 + r, q = q0 + r;
+      px[i] = (p < P && kk < d) ? *(const float4 *)(x + (size_t)p * d + kk) : float4{0.f, 0.f, 0.f, 0.f};
+      py[i] = (q < Q && kk < d) ? *(const float4 *)(y + (size_t)q * d + kk) : float4{0.f, 0.f, 0.f, 0.f};
     }
-    for (int e = tid; e < TN * KC; e += NTH) {
-      const int r = e / KC, k = e - (e / KC) * KC;
-      const int q = q0 + r, kk = k0 + k;
-      Ys[r * LDA + k] = (q < Q && kk < d) ? y[(size_t)q * d + kk] : 0.f;
+  };
+  if constexpr (VEC) fetch(0);
+  for (int k0 = 0; k0 < d; k0 += KC) {
+    if constexpr (VEC) {
+#pragma unroll
+      for (int i = 0; i < NV; ++i) {
+        const int g = tid + i * NTH, r = g >> 3, k = (g & 7) * 4;
+        *(float4 *)(Xs + r * LDA + k) = px[i];
+        *(float4 *)(Ys + r * LDA + k) = py[i];
+      }
+    } else {
+      // stage: each thread loads 16 floats of x and 16 of y (rows of 32 k)
+      for (int e = tid; e < TM * KC; e += NTH) {
+        const int r = e / KC, k = e - (e / KC) * KC;
+        const int p = p0 + r, kk = k0 + k;
+        Xs[r * LDA + k] = (p < P && kk < d) ? x[(size_t)p * d + kk] : 0.f;
+      }
+      for (int e = tid; e < TN * KC; e += NTH) {
+        const int r = e / KC, k = e - (e / KC) * KC;
+        const int q = q0 + r, kk = k0 + k;
+        Ys[r * LDA + k] = (q < Q && kk < d) ? y[(size_t)q * d + kk] : 0.f;
+      }
     }
     __syncthreads();
+    if constexpr (VEC)
+      if (k0 + KC < d) fetch(k0 + KC);  // lands while the matrix cores run
     if constexpr (!BF16) {
 #pragma unroll
       for (int ks = 0; ks < KC; ks += 2) {
@@ -174,8 +204,11 @@ int d2mi_pdist2_mfma_dev(hipStream_t S, int P, int Q, int d, const float *x, con
     d2mi_set_error("pdist2 table of %d x %d too large", P, Q);
     return D2MI_EUNSUPPORTED;
   }
-  if (bf16) k_pdist2_mfma<true><<<(unsigned)tiles, NTH, 0, S>>>(x, y, P, Q, d, nx, ny, T);
-  else k_pdist2_mfma<false><<<(unsigned)tiles, NTH, 0, S>>>(x, y, P, Q, d, nx, ny, T);
+  const bool vec = d % 4 == 0 && ((uintptr_t)x % 16 == 0) && ((uintptr_t)y % 16 == 0);
+  if (bf16 && vec) k_pdist2_mfma<true, true><<<(unsigned)tiles, NTH, 0, S>>>(x, y, P, Q, d, nx, ny, T);
+  else if (bf16) k_pdist2_mfma<true, false><<<(unsigned)tiles, NTH, 0, S>>>(x, y, P, Q, d, nx, ny, T);
+  else if (vec) k_pdist2_mfma<false, true><<<(unsigned)tiles, NTH, 0, S>>>(x, y, P, Q, d, nx, ny, T);
+  else k_pdist2_mfma<false, false><<<(unsigned)tiles, NTH, 0, S>>>(x, y, P, Q, d, nx, ny, T);
   D2MI_HIP(hipGetLastError());
   return D2MI_OK;
 }
