@@ -42,6 +42,12 @@ namespace lp {
 
 constexpr int WAVE = 64;
 constexpr int MAX_WAVES = 16;
+// Vogel start: rows to re-walk after a column is crossed out are walked by
+// one lane each when m <= D2MI_VOGEL_LANE_CELLS * (rows to re-walk), else by
+// the whole wave one row at a time
+#ifndef D2MI_VOGEL_LANE_CELLS
+#define D2MI_VOGEL_LANE_CELLS 50
+#endif
 constexpr int JB = 4;        // nodes per thread per batch in the pointer-jumping rounds
 constexpr int CPT_MAX = 32;  // cells per thread per pricing block (unrolled)
 
@@ -71,7 +77,7 @@ __host__ __device__ inline int ceil_log2(int x) {
 struct Layout {
   size_t ct, A, B, wa, wb, ids, accA, accB, flow, par, depA, depB, lvl, path, ctl, total;
   int levels;
-  __host__ __device__ Layout(int n_max, int m_max, int dA, int dB, bool table, bool ids_on) {
+  __host__ __device__ __forceinline__ Layout(int n_max, int m_max, int dA, int dB, bool table, bool ids_on) {
     const size_t nn = (size_t)n_max + m_max;
     levels = ceil_log2((int)nn) + 2;
     size_t o = 0;
@@ -343,11 +349,26 @@ __device__ __forceinline__ Top2 wave_top2(Top2 t) {
 // cell as far as supply and demand allow and crosses out exactly one line,
 // so the cells form a spanning tree.  Line minima are kept per line and
 // recomputed only for lines whose two cheapest cells were crossed out.
+//
+// One wave runs every step, so the cost of a step is its dependency chain:
+//   * a line's two cheapest cells are found by one lane walking the line
+//     with two independent running pairs (merged at the end), not by a
+//     wave-wide DPP merge per line; all lines of one side that need it are
+//     walked in parallel, one lane each, after the lines to redo have been
+//     compacted into a list (one pass per 64 of them);
+//   * a row crossed out by a column is re-walked by the whole wave (64 cells
+//     at a time and one DPP merge) when the row is long and few rows need it;
+//   * each line's penalty is kept in its own array, so the step's argmax
+//     reads one value per line.
+// The two cheapest (value, index) pairs of a set do not depend on the order
+// in which the set is walked, so every form gives the same start.
 // Output (LDS arrays idle at this point): cell t = (depA[t], depB[t]) with
 // flow accB[t].  Scratch: accA = remaining supply / demand, flow = line
-// minima (as float pairs), lvl = their cells and the active flags.
-__device__ void vogel_start(const Problem &P, unsigned char *lds, const Layout &L,
-                            const float *ct, bool table) {
+// minima (as float pairs), lvl = their cells and the active flags, par =
+// the penalties, path = the list of lines to redo.
+template <bool TABLE>
+__device__ __attribute__((always_inline)) void vogel_start(const Problem &P, unsigned char *lds, const Layout &L,
+                            const float *ct) {
   if (threadIdx.x >= WAVE) return;
   const int lane = threadIdx.x;
   const int n = P.n, m = P.m, nn = n + m;
@@ -356,20 +377,30 @@ __device__ void vogel_start(const Problem &P, unsigned char *lds, const Layout &
   int *ci = (int *)(lds + L.depA), *cj = (int *)(lds + L.depB);
   float *mv1 = (float *)(lds + L.flow), *mv2 = mv1 + nn;
   short *ma1 = (short *)(lds + L.lvl), *ma2 = ma1 + nn, *act = ma1 + 2 * nn;
+  float *pen = (float *)(lds + L.par);
+  int *redo = (int *)(lds + L.path);
+  // TABLE: the FP32 cost table in LDS; a compile-time choice, so the scans
+  // below are straight-line code whose loads are all in flight at once
   auto cost = [&](int k, int j) -> float {
-    return table ? ct[k + j * n] : (float)P.C(k, j);
+    if constexpr (TABLE) return ct[k + j * n];
+    else return (float)P.C(k, j);
   };
   const Top2 none = {FLT_MAX, FLT_MAX, INT_MAX, INT_MAX};
+  // penalty: gap between the two cheapest active cells (0 with one cell)
   auto store = [&](int v, const Top2 &t) {
     mv1[v] = t.v1;
     ma1[v] = (short)(t.a1 == INT_MAX ? -1 : t.a1);
     mv2[v] = t.v2;
     ma2[v] = (short)(t.a2 == INT_MAX ? -1 : t.a2);
+    pen[v] = t.a2 == INT_MAX ? 0.0f : t.v2 - t.v1;
+  };
+  auto merge = [&](Top2 a, const Top2 &b) {
+    return top2_insert(top2_insert(a, b.v1, b.a1), b.v2, b.a2);
   };
   // Scans load unconditionally (addresses stay in range) so the LDS reads
   // of consecutive cells overlap instead of waiting on each other.
   // row k over the active columns, by the whole wave
-  auto row_top2 = [&](int k) {
+  auto row_top2_wave = [&](int k) {
     Top2 t = none;
     for (int j = lane; j < m; j += WAVE) {
       const float c = cost(k, j);
@@ -379,27 +410,51 @@ __device__ void vogel_start(const Problem &P, unsigned char *lds, const Layout &
     t = wave_top2(t);
     if (lane == 0) store(k, t);
   };
-  // column j over the active rows, by one lane
-  auto col_top2 = [&](int j) {
-    Top2 t = none;
-    int k = 0;
-    for (; k + 8 <= n; k += 8) {
-      float c[8];
-      bool on[8];
+  // a line walked by this lane alone: VB cells per batch (every load of the
+  // batch issued before any is used), NP independent running pairs; cells
+  // priced on the fly (no table) are computed one at a time
+  constexpr int VB = TABLE ? 16 : 1, NP = TABLE ? 4 : 1;
+  auto walk = [&](auto cell_cost, auto cell_on, int len) -> Top2 {
+    Top2 tp[NP];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        c[u] = cost(k + u, j);
-        on[u] = act[k + u] != 0;
+    for (int r = 0; r < NP; ++r) tp[r] = none;
+    int i = 0;
+    for (; i + VB <= len; i += VB) {
+      float c[VB];
+      bool on[VB];
+#pragma unroll
+      for (int u = 0; u < VB; ++u) {
+        c[u] = cell_cost(i + u);
+        on[u] = cell_on(i + u);
       }
 #pragma unroll
-      for (int u = 0; u < 8; ++u) t = top2_insert(t, on[u] ? c[u] : FLT_MAX, on[u] ? k + u : INT_MAX);
+      for (int u = 0; u < VB; ++u)
+        tp[u % NP] = top2_insert(tp[u % NP], on[u] ? c[u] : FLT_MAX, on[u] ? i + u : INT_MAX);
     }
-    for (; k < n; ++k) {
-      const float c = cost(k, j);
-      const bool on = act[k] != 0;
-      t = top2_insert(t, on ? c : FLT_MAX, on ? k : INT_MAX);
+    for (; i < len; ++i) {
+      const float c = cell_cost(i);
+      const bool on = cell_on(i);
+      tp[0] = top2_insert(tp[0], on ? c : FLT_MAX, on ? i : INT_MAX);
     }
-    store(n + j, t);
+#pragma unroll
+    for (int r = 1; r < NP; ++r) tp[0] = merge(tp[0], tp[r]);
+    return tp[0];
+  };
+  // row k over the active columns, by this lane alone
+  auto row_top2_lane = [&](int k) {
+    store(k, walk([&](int j) { return cost(k, j); }, [&](int j) { return act[n + j] != 0; }, m));
+  };
+  // column j over the active rows, by this lane alone
+  auto col_top2 = [&](int j) {
+    store(n + j, walk([&](int k) { return cost(k, j); }, [&](int k) { return act[k] != 0; }, n));
+  };
+  // appends the flagged lines of one 64-line slice to redo[]; returns the new count
+  auto append = [&](bool flag, int v, int cnt) {
+    const unsigned long long mask = __ballot(flag);
+    const int below = __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
+                                                 __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+    if (flag) redo[cnt + below] = v;
+    return cnt + __popcll(mask);
   };
   // supplies and demands; the last column takes the remainder (its
   // constraint is the one the LP drops)
@@ -411,21 +466,28 @@ __device__ void vogel_start(const Problem &P, unsigned char *lds, const Layout &
   if (lane == 0) rem[n + m - 1] = sa - sbm > 0.0 ? sa - sbm : 0.0;
   for (int v = lane; v < nn; v += WAVE) act[v] = 1;
   __builtin_amdgcn_wave_barrier();
-  for (int k = 0; k < n; ++k) row_top2(k);
+  for (int k = lane; k < n; k += WAVE) row_top2_lane(k);
   for (int j = lane; j < m; j += WAVE) col_top2(j);
   __builtin_amdgcn_wave_barrier();
   int nr = n, nc = m;
   for (int t = 0; t < nn - 1; ++t) {
-    // line with the largest penalty
+    // line with the largest penalty (crossed-out lines hold -2)
     float bp = -1.0f;
     int bl = INT_MAX;
-    for (int v = lane; v < nn; v += WAVE) {
-      const bool on = act[v] != 0;
-      const float p2 = mv2[v], p1 = mv1[v];
-      const float pen = !on ? -2.0f : (ma2[v] >= 0 ? p2 - p1 : 0.0f);
-      const bool better = pen > bp || (pen == bp && v < bl);
-      bp = better ? pen : bp;
-      bl = better ? v : bl;
+    for (int v0 = 0; v0 < nn; v0 += 8 * WAVE) {
+      float pv[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int v = v0 + u * WAVE + lane;
+        pv[u] = pen[v < nn ? v : 0];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int v = v0 + u * WAVE + lane;
+        const bool better = v < nn && (pv[u] > bp || (pv[u] == bp && v < bl));
+        bp = better ? pv[u] : bp;
+        bl = better ? v : bl;
+      }
     }
     // wave argmax (largest penalty, lowest line on ties), DPP as wave_min_d
     auto pen_merge = [&](float op, int ol) {
@@ -449,30 +511,43 @@ __device__ void vogel_start(const Problem &P, unsigned char *lds, const Layout &
     const bool cross_row = (ra - q <= rb - q && nr > 1) || nc == 1;
     __builtin_amdgcn_wave_barrier();
     if (lane == 0) {
+      const int gone = cross_row ? is : n + js;
       rem[is] = ra - q;
       rem[n + js] = rb - q;
       ci[t] = is;
       cj[t] = js;
       cq[t] = q;
-      act[cross_row ? is : n + js] = 0;
+      act[gone] = 0;
+      pen[gone] = -2.0f;
     }
     __builtin_amdgcn_wave_barrier();
     if (t == nn - 2) break;
+    int cnt = 0;
     if (cross_row) {
       --nr;
-      for (int j = lane; j < m; j += WAVE)
-        if (act[n + j] && (ma1[n + j] == is || ma2[n + j] == is)) col_top2(j);
+      // columns whose two cheapest cells included row is, one lane each
+      for (int j0 = 0; j0 < m; j0 += WAVE) {
+        const int j = j0 + lane;
+        const bool aff = j < m && act[n + j] && (ma1[n + j] == is || ma2[n + j] == is);
+        cnt = append(aff, j, cnt);
+      }
+      __builtin_amdgcn_wave_barrier();
+      for (int r = lane; r < cnt; r += WAVE) col_top2(redo[r]);
     } else {
       --nc;
-      for (int b = 0; b < n; b += WAVE) {
-        const int k = b + lane;
+      // rows whose two cheapest cells included column js
+      for (int k0 = 0; k0 < n; k0 += WAVE) {
+        const int k = k0 + lane;
         const bool aff = k < n && act[k] && (ma1[k] == js || ma2[k] == js);
-        unsigned long long mask = __ballot(aff);
-        while (mask) {
-          const int bit = __ffsll((long long)mask) - 1;
-          row_top2(b + bit);
-          mask &= mask - 1;
-        }
+        cnt = append(aff, k, cnt);
+      }
+      __builtin_amdgcn_wave_barrier();
+      // one pass of lanes walking m cells each, or one wave-wide walk and
+      // DPP merge per row
+      if (m <= D2MI_VOGEL_LANE_CELLS * cnt) {
+        for (int r = lane; r < cnt; r += WAVE) row_top2_lane(redo[r]);
+      } else {
+        for (int r = 0; r < cnt; ++r) row_top2_wave(redo[r]);
       }
     }
     __builtin_amdgcn_wave_barrier();
@@ -482,7 +557,7 @@ __device__ void vogel_start(const Problem &P, unsigned char *lds, const Layout &
 // Solves one LP with the whole workgroup (blockDim.x a multiple of 64, at
 // most 64*MAX_WAVES threads); returns the optimal value to all threads.
 // status bit 0 is set when the pivot limit is reached.
-__device__ double solve(const Problem &P, unsigned char *lds, const Layout &L, int *pivots_out,
+__device__ __attribute__((always_inline)) double solve(const Problem &P, unsigned char *lds, const Layout &L, int *pivots_out,
                         int *status) {
   const int tid = threadIdx.x, T = blockDim.x, nw = T / WAVE;
   const int lane = tid & (WAVE - 1);
@@ -534,7 +609,8 @@ __device__ double solve(const Problem &P, unsigned char *lds, const Layout &L, i
   // north-west corner), else the north-west corner
   bool have_start = false;
   if (P.vogel && n > 1 && m > 1) {
-    vogel_start(P, lds, L, ct, table);
+    if (table) vogel_start<true>(P, lds, L, ct);
+    else vogel_start<false>(P, lds, L, ct);
     __syncthreads();
     // root the basis cells at row 0: a node whose tree neighbour is already
     // rooted hangs under it (each node is reached exactly once in a tree)
