@@ -675,13 +675,40 @@ __device__ __attribute__((always_inline)) double solve(const Problem &P, unsigne
   int *dep = depA;
   for (;;) {
     // ---- pointer jumping: potentials, depths, lifting table
-    for (int v = tid; v < nn; v += T) {
-      const int p = par[v];
-      lvl[v] = (short)p;
-      double c = 0.0;
-      if (p >= 0) c = exact_mode ? C(edge_k(v, p), edge_j(v, p)) : (double)ct[edge_cell(v, p)];
-      accA[v] = c;
-      depA[v] = p >= 0 ? 1 : 0;
+    // JB nodes per thread at a time: parents, then edge costs, then stores
+    for (int v0 = tid; v0 < nn; v0 += JB * T) {
+      int pp[JB];
+      double cc[JB];
+#pragma unroll
+      for (int u = 0; u < JB; ++u) {
+        const int v = v0 + u * T;
+        pp[u] = v < nn ? par[v] : -1;
+      }
+      if (exact_mode) {
+#pragma unroll
+        for (int u = 0; u < JB; ++u) {
+          const int v = v0 + u * T;
+          cc[u] = pp[u] >= 0 ? C(edge_k(v, pp[u]), edge_j(v, pp[u])) : 0.0;
+        }
+      } else {
+        float cf[JB];
+#pragma unroll
+        for (int u = 0; u < JB; ++u) {
+          const int v = v0 + u * T;
+          cf[u] = ct[pp[u] >= 0 ? edge_cell(v, pp[u]) : 0];
+        }
+#pragma unroll
+        for (int u = 0; u < JB; ++u) cc[u] = pp[u] >= 0 ? (double)cf[u] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < JB; ++u) {
+        const int v = v0 + u * T;
+        if (v < nn) {
+          lvl[v] = (short)pp[u];
+          accA[v] = cc[u];
+          depA[v] = pp[u] >= 0 ? 1 : 0;
+        }
+      }
     }
     __syncthreads();
     {
@@ -747,20 +774,43 @@ __device__ __attribute__((always_inline)) double solve(const Problem &P, unsigne
       int k, j;
       split_cell(e, n, inv_n, k, j);
       if (table && !exact_mode) {
+        // PB cells at a time: their indices first, then every load of the
+        // batch (cells past the block read cell 0), then the same compares in
+        // the same order as one cell at a time
+        constexpr int PB = 8;
+        for (int i0 = 0; i0 < P.cpt; i0 += PB) {
+          int ee[PB], kk[PB], jj[PB];
+          bool ok[PB];
 #pragma unroll
-        for (int i = 0; i < CPT_MAX; ++i) {
-          if (i < P.cpt && scanned + tid + i * T < N) {
-            const double r = (double)ct[e] - pot[k] - pot[n + j];
-            if (bland) {
-              if (r < -eps && e < bestc) { bestc = e; best = r; }
-            } else if (r < best || (r == best && bestc != INT_MAX && e < bestc)) {
-              best = r;
-              bestc = e;
+          for (int u = 0; u < PB; ++u) {
+            ok[u] = i0 + u < P.cpt && scanned + tid + (i0 + u) * T < N;
+            ee[u] = ok[u] ? e : 0;
+            kk[u] = ok[u] ? k : 0;
+            jj[u] = ok[u] ? j : 0;
+            e += T; k += Tr; j += Tq;
+            if (k >= n) { k -= n; ++j; }
+            if (e >= N) { e -= N; j -= m; }
+          }
+          float cv[PB];
+          double pu[PB], pv[PB];
+#pragma unroll
+          for (int u = 0; u < PB; ++u) {
+            cv[u] = ct[ee[u]];
+            pu[u] = pot[kk[u]];
+            pv[u] = pot[n + jj[u]];
+          }
+#pragma unroll
+          for (int u = 0; u < PB; ++u) {
+            if (ok[u]) {
+              const double r = (double)cv[u] - pu[u] - pv[u];
+              if (bland) {
+                if (r < -eps && ee[u] < bestc) { bestc = ee[u]; best = r; }
+              } else if (r < best || (r == best && bestc != INT_MAX && ee[u] < bestc)) {
+                best = r;
+                bestc = ee[u];
+              }
             }
           }
-          e += T; k += Tr; j += Tq;
-          if (k >= n) { k -= n; ++j; }
-          if (e >= N) { e -= N; j -= m; }
         }
       } else {
         for (int i = 0; i < P.cpt && scanned + tid + i * T < N; ++i) {
