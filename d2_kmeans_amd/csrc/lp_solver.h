@@ -45,6 +45,10 @@ constexpr int MAX_WAVES = 16;
 // Vogel start: rows to re-walk after a column is crossed out are walked by
 // one lane each when m <= D2MI_VOGEL_LANE_CELLS * (rows to re-walk), else by
 // the whole wave one row at a time
+// pointer jumping by wave 0 alone for trees of at most this many nodes
+#ifndef D2MI_LP_SOLO_NODES
+#define D2MI_LP_SOLO_NODES 256
+#endif
 #ifndef D2MI_VOGEL_LANE_CELLS
 #define D2MI_VOGEL_LANE_CELLS 50
 #endif
@@ -226,7 +230,9 @@ struct Ctl {
   double rv[2][MAX_WAVES];
   int ri[2][MAX_WAVES];
   int flag[2][MAX_WAVES];
+  int jump_b;  // pointer jumping by one wave: 1 when its results are in the B buffers
 };
+static_assert(sizeof(Ctl) <= 1024, "the layout reserves 1024 B for the control block");
 
 // Workgroup argmin of (v, i) lexicographically; every thread gets the result.
 __device__ __forceinline__ void block_argmin(double &v, int &i, Ctl *ctl, int &bank, int nw) {
@@ -674,44 +680,61 @@ __device__ __attribute__((always_inline)) double solve(const Problem &P, unsigne
   double *pot = accA;
   int *dep = depA;
   for (;;) {
-    // ---- pointer jumping: potentials, depths, lifting table
-    // JB nodes per thread at a time: parents, then edge costs, then stores
-    for (int v0 = tid; v0 < nn; v0 += JB * T) {
-      int pp[JB];
-      double cc[JB];
-#pragma unroll
-      for (int u = 0; u < JB; ++u) {
-        const int v = v0 + u * T;
-        pp[u] = v < nn ? par[v] : -1;
-      }
-      if (exact_mode) {
+    // ---- pointer jumping: potentials, depths, lifting table.  A tree of at
+    // most D2MI_LP_SOLO_NODES nodes is done by wave 0 alone: its rounds are
+    // ordered by the wave itself (LDS operations of one wave complete in
+    // order), so a round costs two LDS round trips and no workgroup barrier;
+    // the other waves wait at one barrier at the end.  Larger trees use every
+    // wave and a workgroup barrier per round.
+    const bool solo = nn <= D2MI_LP_SOLO_NODES;
+    if (!solo || wid == 0) {
+      const int jt = solo ? lane : tid, JT = solo ? WAVE : T;
+      auto round_sync = [&]() {
+        if (solo) {
+          asm volatile("" ::: "memory");
+          __builtin_amdgcn_wave_barrier();
+          asm volatile("" ::: "memory");
+        } else {
+          __syncthreads();
+        }
+      };
+      // JB nodes per thread at a time: parents, then edge costs, then stores
+      for (int v0 = jt; v0 < nn; v0 += JB * JT) {
+        int pp[JB];
+        double cc[JB];
 #pragma unroll
         for (int u = 0; u < JB; ++u) {
-          const int v = v0 + u * T;
-          cc[u] = pp[u] >= 0 ? C(edge_k(v, pp[u]), edge_j(v, pp[u])) : 0.0;
+          const int v = v0 + u * JT;
+          const int p = par[min(v, nn - 1)];  // loads stay in range, no branch
+          pp[u] = v < nn ? p : -1;
         }
-      } else {
-        float cf[JB];
+        if (exact_mode) {
+#pragma unroll
+          for (int u = 0; u < JB; ++u) {
+            const int v = v0 + u * JT;
+            cc[u] = pp[u] >= 0 ? C(edge_k(v, pp[u]), edge_j(v, pp[u])) : 0.0;
+          }
+        } else {
+          float cf[JB];
+#pragma unroll
+          for (int u = 0; u < JB; ++u) {
+            const int v = v0 + u * JT;
+            cf[u] = ct[pp[u] >= 0 ? edge_cell(v, pp[u]) : 0];
+          }
+#pragma unroll
+          for (int u = 0; u < JB; ++u) cc[u] = pp[u] >= 0 ? (double)cf[u] : 0.0;
+        }
 #pragma unroll
         for (int u = 0; u < JB; ++u) {
-          const int v = v0 + u * T;
-          cf[u] = ct[pp[u] >= 0 ? edge_cell(v, pp[u]) : 0];
-        }
-#pragma unroll
-        for (int u = 0; u < JB; ++u) cc[u] = pp[u] >= 0 ? (double)cf[u] : 0.0;
-      }
-#pragma unroll
-      for (int u = 0; u < JB; ++u) {
-        const int v = v0 + u * T;
-        if (v < nn) {
-          lvl[v] = (short)pp[u];
-          accA[v] = cc[u];
-          depA[v] = pp[u] >= 0 ? 1 : 0;
+          const int v = v0 + u * JT;
+          if (v < nn) {
+            lvl[v] = (short)pp[u];
+            accA[v] = cc[u];
+            depA[v] = pp[u] >= 0 ? 1 : 0;
+          }
         }
       }
-    }
-    __syncthreads();
-    {
+      round_sync();
       double *aS = accA, *aD = accB;
       int *dS = depA, *dD = depB;
       for (int r = 0; r + 1 < L.levels; ++r) {
@@ -720,21 +743,25 @@ __device__ __attribute__((always_inline)) double solve(const Problem &P, unsigne
         const double sg = r == 0 ? -1.0 : 1.0;
         int any = 0;
         // JB nodes per thread at a time: every load of the batch is issued
-        // before its stores, so the LDS latencies overlap
-        for (int v0 = tid; v0 < nn; v0 += JB * T) {
+        // before its stores, so the LDS latencies overlap; loads read clamped
+        // indices unconditionally (a guarded load is a branch, and branches,
+        // not LDS latency, set the cost of a round on one wave)
+        for (int v0 = jt; v0 < nn; v0 += JB * JT) {
           int vv[JB], a[JB], aa[JB], dv[JB], da[JB];
           double av[JB], aav[JB];
 #pragma unroll
           for (int u = 0; u < JB; ++u) {
-            vv[u] = v0 + u * T;
-            a[u] = vv[u] < nn ? cur[vv[u]] : -1;
-            av[u] = vv[u] < nn ? aS[vv[u]] : 0.0;
-            dv[u] = vv[u] < nn ? dS[vv[u]] : 0;
+            vv[u] = v0 + u * JT;
+            const int vc = min(vv[u], nn - 1);
+            a[u] = cur[vc];
+            av[u] = aS[vc];
+            dv[u] = dS[vc];
           }
 #pragma unroll
           for (int u = 0; u < JB; ++u) {
             const int x = a[u] >= 0 ? a[u] : 0;
-            aa[u] = a[u] >= 0 ? cur[x] : -1;
+            const int ax = cur[x];
+            aa[u] = a[u] >= 0 ? ax : -1;
             aav[u] = aS[x];
             da[u] = dS[x];
           }
@@ -749,13 +776,25 @@ __device__ __attribute__((always_inline)) double solve(const Problem &P, unsigne
             }
           }
         }
-        any = block_or(any, ctl, bank, nw);
+        if (solo) {
+          any = __any(any) ? 1 : 0;
+          round_sync();
+        } else {
+          any = block_or(any, ctl, bank, nw);
+        }
         double *td = aS; aS = aD; aD = td;
         int *ti = dS; dS = dD; dD = ti;
         if (!any) break;
       }
+      if (solo && lane == 0) ctl->jump_b = aS == accB ? 1 : 0;
       pot = aS;
       dep = dS;
+    }
+    if (solo) {
+      __syncthreads();
+      const bool in_b = ctl->jump_b != 0;
+      pot = in_b ? accB : accA;
+      dep = in_b ? depB : depA;
     }
     double *pfl = pot == accA ? accB : accA;  // free during the pivot
 
