@@ -838,16 +838,23 @@ __device__ __attribute__((always_inline)) double solve(const Problem &P, unsigne
             pu[u] = pot[kk[u]];
             pv[u] = pot[n + jj[u]];
           }
+          // selects, not branches: the same decisions as one cell at a time
+          if (bland) {
 #pragma unroll
-          for (int u = 0; u < PB; ++u) {
-            if (ok[u]) {
+            for (int u = 0; u < PB; ++u) {
               const double r = (double)cv[u] - pu[u] - pv[u];
-              if (bland) {
-                if (r < -eps && ee[u] < bestc) { bestc = ee[u]; best = r; }
-              } else if (r < best || (r == best && bestc != INT_MAX && ee[u] < bestc)) {
-                best = r;
-                bestc = ee[u];
-              }
+              const bool take = ok[u] && r < -eps && ee[u] < bestc;
+              best = take ? r : best;
+              bestc = take ? ee[u] : bestc;
+            }
+          } else {
+#pragma unroll
+            for (int u = 0; u < PB; ++u) {
+              const double r = (double)cv[u] - pu[u] - pv[u];
+              const bool take =
+                  ok[u] && (r < best || (r == best && bestc != INT_MAX && ee[u] < bestc));
+              best = take ? r : best;
+              bestc = take ? ee[u] : bestc;
             }
           }
         }
