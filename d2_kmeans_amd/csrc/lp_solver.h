@@ -37,20 +37,18 @@
 #include <cfloat>
 #include <climits>
 #include <cmath>
+#include <type_traits>
 
 namespace lp {
 
 constexpr int WAVE = 64;
 constexpr int MAX_WAVES = 16;
 // Vogel start: rows to re-walk after a column is crossed out are walked by
-// one lane each when m <= D2MI_VOGEL_LANE_CELLS * (rows to re-walk), else by
-// the whole wave one row at a time
+// groups of 64 / cnt lanes (2 to 16) when there are cnt <= 32 of them, by one
+// lane each beyond that, and one long row alone by the whole wave
 // pointer jumping by wave 0 alone for trees of at most this many nodes
 #ifndef D2MI_LP_SOLO_NODES
 #define D2MI_LP_SOLO_NODES 256
-#endif
-#ifndef D2MI_VOGEL_LANE_CELLS
-#define D2MI_VOGEL_LANE_CELLS 50
 #endif
 constexpr int JB = 4;        // nodes per thread per batch in the pointer-jumping rounds
 constexpr int CPT_MAX = 32;  // cells per thread per pricing block (unrolled)
@@ -159,7 +157,10 @@ __device__ __forceinline__ void split_cell(int e, int n, float inv_n, int &k, in
 // as a scalar.  Selections are exact, so the order does not matter: results
 // equal those of any other order.  All 64 lanes must be active.
 enum { DPP_ROR1 = 0x121, DPP_ROR2 = 0x122, DPP_ROR4 = 0x124, DPP_ROR8 = 0x128,
-       DPP_BC15 = 0x142, DPP_BC31 = 0x143 };
+       DPP_BC15 = 0x142, DPP_BC31 = 0x143,
+       // lane groups: quad_perm [1,0,3,2] and [2,3,0,1] (lane xor 1, xor 2),
+       // lane i <- 7 - i within each 8 lanes, lane i <- 15 - i within each 16
+       DPP_QX1 = 0xB1, DPP_QX2 = 0x4E, DPP_HMIRROR = 0x141, DPP_MIRROR = 0x140 };
 template <int CTRL, int ROWS = 0xf>
 __device__ __forceinline__ int dpp_i(int v, int id = 0) {
   return __builtin_amdgcn_update_dpp(id, v, CTRL, ROWS, 0xf, false);
@@ -362,8 +363,11 @@ __device__ __forceinline__ Top2 wave_top2(Top2 t) {
 //     wave-wide DPP merge per line; all lines of one side that need it are
 //     walked in parallel, one lane each, after the lines to redo have been
 //     compacted into a list (one pass per 64 of them);
-//   * a row crossed out by a column is re-walked by the whole wave (64 cells
-//     at a time and one DPP merge) when the row is long and few rows need it;
+//   * rows whose two cheapest cells a crossed-out column held are re-walked
+//     by groups of 64 / cnt lanes each (cnt rows, all at once; 2 to 16 lanes
+//     per row, each lane a stride of the row, pairs merged by DPP within the
+//     group), so one VALU pass covers m / group cells instead of m: a lone
+//     wave issues each instruction over 4 cycles, and the walk is its work;
 //   * each line's penalty is kept in its own array, so the step's argmax
 //     reads one value per line.
 // The two cheapest (value, index) pairs of a set do not depend on the order
@@ -449,6 +453,27 @@ __device__ __attribute__((always_inline)) void vogel_start(const Problem &P, uns
   // row k over the active columns, by this lane alone
   auto row_top2_lane = [&](int k) {
     store(k, walk([&](int j) { return cost(k, j); }, [&](int j) { return act[n + j] != 0; }, m));
+  };
+  // the cnt rows of redo[] at once, G lanes per row (G * cnt <= 64): lane l
+  // walks columns q, q + G, ... (q = l % G) of row redo[l / G]; the walk's
+  // order of (value, position) is that of (value, column), and the G pairs of
+  // a group are merged by DPP within it (quad permutes, then the half-row and
+  // row mirrors: every pair merged exactly once).  Lanes past the list repeat
+  // its last row and store nothing.
+  auto rows_top2_group = [&](auto gc, int cnt) {
+    constexpr int G = decltype(gc)::value;
+    const int q = lane & (G - 1), slot = lane / G;
+    const int k = redo[slot < cnt ? slot : cnt - 1];
+    const int len = (m - q + G - 1) / G;  // 0 when q >= m
+    Top2 t = walk([&](int i) { return cost(k, q + i * G); },
+                  [&](int i) { return act[n + q + i * G] != 0; }, len);
+    if (t.a1 != INT_MAX) t.a1 = q + t.a1 * G;
+    if (t.a2 != INT_MAX) t.a2 = q + t.a2 * G;
+    if constexpr (G >= 2) t = top2_merge_dpp<DPP_QX1>(t);
+    if constexpr (G >= 4) t = top2_merge_dpp<DPP_QX2>(t);
+    if constexpr (G >= 8) t = top2_merge_dpp<DPP_HMIRROR>(t);
+    if constexpr (G >= 16) t = top2_merge_dpp<DPP_MIRROR>(t);
+    if (q == 0 && slot < cnt) store(k, t);
   };
   // column j over the active rows, by this lane alone
   auto col_top2 = [&](int j) {
@@ -548,13 +573,16 @@ __device__ __attribute__((always_inline)) void vogel_start(const Problem &P, uns
         cnt = append(aff, k, cnt);
       }
       __builtin_amdgcn_wave_barrier();
-      // one pass of lanes walking m cells each, or one wave-wide walk and
-      // DPP merge per row
-      if (m <= D2MI_VOGEL_LANE_CELLS * cnt) {
+      // one row: the whole wave if it is long; else every row at once, by
+      // groups of 64 / cnt lanes (16 at most), or one lane each past 32 rows
+      if (cnt == 1 && m >= 128) row_top2_wave(redo[0]);
+      else if (cnt == 0) {}
+      else if (cnt <= 4) rows_top2_group(std::integral_constant<int, 16>{}, cnt);
+      else if (cnt <= 8) rows_top2_group(std::integral_constant<int, 8>{}, cnt);
+      else if (cnt <= 16) rows_top2_group(std::integral_constant<int, 4>{}, cnt);
+      else if (cnt <= 32) rows_top2_group(std::integral_constant<int, 2>{}, cnt);
+      else
         for (int r = lane; r < cnt; r += WAVE) row_top2_lane(redo[r]);
-      } else {
-        for (int r = 0; r < cnt; ++r) row_top2_wave(redo[r]);
-      }
     }
     __builtin_amdgcn_wave_barrier();
   }
