@@ -358,16 +358,14 @@ __device__ __forceinline__ Top2 wave_top2(Top2 t) {
 // recomputed only for lines whose two cheapest cells were crossed out.
 //
 // One wave runs every step, so the cost of a step is its dependency chain:
-//   * a line's two cheapest cells are found by one lane walking the line
-//     with two independent running pairs (merged at the end), not by a
-//     wave-wide DPP merge per line; all lines of one side that need it are
-//     walked in parallel, one lane each, after the lines to redo have been
-//     compacted into a list (one pass per 64 of them);
-//   * rows whose two cheapest cells a crossed-out column held are re-walked
-//     by groups of 64 / cnt lanes each (cnt rows, all at once; 2 to 16 lanes
-//     per row, each lane a stride of the row, pairs merged by DPP within the
-//     group), so one VALU pass covers m / group cells instead of m: a lone
-//     wave issues each instruction over 4 cycles, and the walk is its work;
+//   * the lines to redo after a step (those whose two cheapest cells the
+//     crossed-out line held) are compacted into a list, and all cnt of them
+//     are walked at once: by groups of 64 / cnt lanes each when cnt <= 32
+//     (2 to 16 lanes per line, each lane a stride of the line with
+//     independent running pairs, the group's pairs merged by DPP), else one
+//     lane per line; a lone long row goes to the whole wave.  One VALU pass
+//     then covers len / group cells instead of len: a lone wave issues each
+//     instruction over 4 cycles, and the walks are most of its work;
 //   * each line's penalty is kept in its own array, so the step's argmax
 //     reads one value per line.
 // The two cheapest (value, index) pairs of a set do not depend on the order
@@ -454,30 +452,44 @@ __device__ __attribute__((always_inline)) void vogel_start(const Problem &P, uns
   auto row_top2_lane = [&](int k) {
     store(k, walk([&](int j) { return cost(k, j); }, [&](int j) { return act[n + j] != 0; }, m));
   };
-  // the cnt rows of redo[] at once, G lanes per row (G * cnt <= 64): lane l
-  // walks columns q, q + G, ... (q = l % G) of row redo[l / G]; the walk's
-  // order of (value, position) is that of (value, column), and the G pairs of
-  // a group are merged by DPP within it (quad permutes, then the half-row and
-  // row mirrors: every pair merged exactly once).  Lanes past the list repeat
-  // its last row and store nothing.
-  auto rows_top2_group = [&](auto gc, int cnt) {
+  // column j over the active rows, by this lane alone
+  auto col_top2 = [&](int j) {
+    store(n + j, walk([&](int k) { return cost(k, j); }, [&](int k) { return act[k] != 0; }, n));
+  };
+  // the cnt lines of redo[] (rows, or columns when COL) at once, G lanes per
+  // line (G * cnt <= 64): lane l walks cells q, q + G, ... (q = l % G) of line
+  // redo[l / G]; the walk's order of (value, position) is that of (value,
+  // cell), and the G pairs of a group are merged by DPP within it (quad
+  // permutes, then the half-row and row mirrors: every pair merged exactly
+  // once).  Lanes past the list repeat its last line and store nothing.
+  auto lines_top2_group = [&](auto gc, auto colc, int cnt) {
     constexpr int G = decltype(gc)::value;
+    constexpr bool COL = decltype(colc)::value;
     const int q = lane & (G - 1), slot = lane / G;
-    const int k = redo[slot < cnt ? slot : cnt - 1];
-    const int len = (m - q + G - 1) / G;  // 0 when q >= m
-    Top2 t = walk([&](int i) { return cost(k, q + i * G); },
-                  [&](int i) { return act[n + q + i * G] != 0; }, len);
+    const int x = redo[slot < cnt ? slot : cnt - 1];
+    const int len = ((COL ? n : m) - q + G - 1) / G;  // 0 when q is past the line
+    Top2 t = walk([&](int i) { return COL ? cost(q + i * G, x) : cost(x, q + i * G); },
+                  [&](int i) { return act[(COL ? 0 : n) + q + i * G] != 0; }, len);
     if (t.a1 != INT_MAX) t.a1 = q + t.a1 * G;
     if (t.a2 != INT_MAX) t.a2 = q + t.a2 * G;
     if constexpr (G >= 2) t = top2_merge_dpp<DPP_QX1>(t);
     if constexpr (G >= 4) t = top2_merge_dpp<DPP_QX2>(t);
     if constexpr (G >= 8) t = top2_merge_dpp<DPP_HMIRROR>(t);
     if constexpr (G >= 16) t = top2_merge_dpp<DPP_MIRROR>(t);
-    if (q == 0 && slot < cnt) store(k, t);
+    if (q == 0 && slot < cnt) store(COL ? n + x : x, t);
   };
-  // column j over the active rows, by this lane alone
-  auto col_top2 = [&](int j) {
-    store(n + j, walk([&](int k) { return cost(k, j); }, [&](int k) { return act[k] != 0; }, n));
+  // every line of redo[]: groups of 64 / cnt lanes (16 at most) per line, or
+  // one lane per line past 32 lines
+  auto lines_top2 = [&](auto colc, int cnt) {
+    if (cnt == 0) return;
+    if (cnt <= 4) lines_top2_group(std::integral_constant<int, 16>{}, colc, cnt);
+    else if (cnt <= 8) lines_top2_group(std::integral_constant<int, 8>{}, colc, cnt);
+    else if (cnt <= 16) lines_top2_group(std::integral_constant<int, 4>{}, colc, cnt);
+    else if (cnt <= 32) lines_top2_group(std::integral_constant<int, 2>{}, colc, cnt);
+    else if constexpr (decltype(colc)::value)
+      for (int r = lane; r < cnt; r += WAVE) col_top2(redo[r]);
+    else
+      for (int r = lane; r < cnt; r += WAVE) row_top2_lane(redo[r]);
   };
   // appends the flagged lines of one 64-line slice to redo[]; returns the new count
   auto append = [&](bool flag, int v, int cnt) {
@@ -556,14 +568,14 @@ __device__ __attribute__((always_inline)) void vogel_start(const Problem &P, uns
     int cnt = 0;
     if (cross_row) {
       --nr;
-      // columns whose two cheapest cells included row is, one lane each
+      // columns whose two cheapest cells included row is
       for (int j0 = 0; j0 < m; j0 += WAVE) {
         const int j = j0 + lane;
         const bool aff = j < m && act[n + j] && (ma1[n + j] == is || ma2[n + j] == is);
         cnt = append(aff, j, cnt);
       }
       __builtin_amdgcn_wave_barrier();
-      for (int r = lane; r < cnt; r += WAVE) col_top2(redo[r]);
+      lines_top2(std::true_type{}, cnt);
     } else {
       --nc;
       // rows whose two cheapest cells included column js
@@ -573,16 +585,9 @@ __device__ __attribute__((always_inline)) void vogel_start(const Problem &P, uns
         cnt = append(aff, k, cnt);
       }
       __builtin_amdgcn_wave_barrier();
-      // one row: the whole wave if it is long; else every row at once, by
-      // groups of 64 / cnt lanes (16 at most), or one lane each past 32 rows
+      // one row: the whole wave if it is long; else every row at once
       if (cnt == 1 && m >= 128) row_top2_wave(redo[0]);
-      else if (cnt == 0) {}
-      else if (cnt <= 4) rows_top2_group(std::integral_constant<int, 16>{}, cnt);
-      else if (cnt <= 8) rows_top2_group(std::integral_constant<int, 8>{}, cnt);
-      else if (cnt <= 16) rows_top2_group(std::integral_constant<int, 4>{}, cnt);
-      else if (cnt <= 32) rows_top2_group(std::integral_constant<int, 2>{}, cnt);
-      else
-        for (int r = lane; r < cnt; r += WAVE) row_top2_lane(redo[r]);
+      else lines_top2(std::false_type{}, cnt);
     }
     __builtin_amdgcn_wave_barrier();
   }
