@@ -323,6 +323,19 @@ __device__ __forceinline__ Top2 top2_insert(Top2 t, float c, int i) {
   return r;
 }
 
+// insert for a walk in increasing index order: the new cell's index is above
+// every index held, so it never wins a tie and (value, index) order reduces
+// to value order (one compare per slot); empty slots hold +inf
+__device__ __forceinline__ Top2 top2_push(Top2 t, float c, int i) {
+  const bool b1 = c < t.v1, b2 = c < t.v2;
+  Top2 r;
+  r.v1 = b1 ? c : t.v1;
+  r.a1 = b1 ? i : t.a1;
+  r.v2 = b1 ? t.v1 : (b2 ? c : t.v2);
+  r.a2 = b1 ? t.a1 : (b2 ? i : t.a2);
+  return r;
+}
+
 // wave-wide merge, every lane gets the result (DPP as wave_min_d: each lane's
 // pair is merged exactly once, and the two smallest (value, index) pairs of
 // distinct cells do not depend on the merge order)
@@ -422,10 +435,15 @@ __device__ __attribute__((always_inline)) void vogel_start(const Problem &P, uns
   // batch issued before any is used), NP independent running pairs; cells
   // priced on the fly (no table) are computed one at a time
   constexpr int VB = TABLE ? 16 : 1, NP = TABLE ? 4 : 1;
+  // Each running pair sees its cells in increasing order (top2_push); an
+  // inactive cell costs +inf and is never taken; empty slots read +inf until
+  // the end, where they become FLT_MAX as everywhere else.  The result equals
+  // top2_insert over the same cells in any order.
   auto walk = [&](auto cell_cost, auto cell_on, int len) -> Top2 {
+    const float inf = __builtin_huge_valf();
     Top2 tp[NP];
 #pragma unroll
-    for (int r = 0; r < NP; ++r) tp[r] = none;
+    for (int r = 0; r < NP; ++r) tp[r] = Top2{inf, inf, INT_MAX, INT_MAX};
     int i = 0;
     for (; i + VB <= len; i += VB) {
       float c[VB];
@@ -437,15 +455,17 @@ __device__ __attribute__((always_inline)) void vogel_start(const Problem &P, uns
       }
 #pragma unroll
       for (int u = 0; u < VB; ++u)
-        tp[u % NP] = top2_insert(tp[u % NP], on[u] ? c[u] : FLT_MAX, on[u] ? i + u : INT_MAX);
+        tp[u % NP] = top2_push(tp[u % NP], on[u] ? c[u] : inf, i + u);
     }
     for (; i < len; ++i) {
       const float c = cell_cost(i);
       const bool on = cell_on(i);
-      tp[0] = top2_insert(tp[0], on ? c : FLT_MAX, on ? i : INT_MAX);
+      tp[0] = top2_push(tp[0], on ? c : inf, i);
     }
 #pragma unroll
     for (int r = 1; r < NP; ++r) tp[0] = merge(tp[0], tp[r]);
+    if (tp[0].a1 == INT_MAX) tp[0].v1 = FLT_MAX;
+    if (tp[0].a2 == INT_MAX) tp[0].v2 = FLT_MAX;
     return tp[0];
   };
   // row k over the active columns, by this lane alone
