@@ -74,10 +74,12 @@ __global__ __launch_bounds__(LB_T) void k_srv_bounds(int K, int s, int d,
                                                      const double *__restrict__ supp,
                                                      double *__restrict__ lb,
                                                      double *__restrict__ vals) {
-  // LDS: s*d centroid supports, 4 waves x LB_KC row minima, d mean shifts
+  // LDS: s*d centroid supports, 4 waves x LB_KC row minima, d mean shifts,
+  // 2 x LB_T partial sums of the means
   extern __shared__ __align__(16) double shA[];
   double *rmin_w = shA + (size_t)s * d;
   double *delta = rmin_w + 4 * LB_KC;
+  double *pmc = delta + d, *pmx = pmc + LB_T;
   __shared__ double red[8];
   const int r = blockIdx.x / K, c = blockIdx.x - (blockIdx.x / K) * K;
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
@@ -86,11 +88,43 @@ __global__ __launch_bounds__(LB_T) void k_srv_bounds(int K, int s, int d,
   const double *A = cs + (size_t)c * s * d;
   const double *aw = cw + (size_t)c * s;
   for (int q = tid; q < s * d; q += LB_T) shA[q] = A[q];
-  // delta = mu_c - mu_x, one dimension per thread
-  for (int t = tid; t < d; t += LB_T) {
+  // delta = mu_c - mu_x.  Every thread takes part: thread (part, t) sums
+  // dimension t over the points part, part + PD, ... (four at a time, so
+  // their loads are in flight together), then each dimension adds its PD
+  // partials in a fixed order (d > LB_T: one dimension per thread, directly)
+  const bool parts = d <= LB_T;
+  const int PD = parts ? LB_T / d : 1;
+  for (int q = tid; q < PD * d; q += LB_T) {
+    const int part = q / d, t = q - part * d;
+    double mc = 0.0, mx[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int k = part; k < s; k += PD) mc += aw[k] * A[(size_t)k * d + t];
+    int j = part;
+    for (; j + 3 * PD < m; j += 4 * PD) {
+      double wv[4], xv[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        wv[u] = w[off + j + u * PD];
+        xv[u] = supp[(size_t)(off + j + u * PD) * d + t];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) mx[u] += wv[u] * xv[u];
+    }
+    for (; j < m; j += PD) mx[0] += w[off + j] * supp[(size_t)(off + j) * d + t];
+    const double sx = (mx[0] + mx[1]) + (mx[2] + mx[3]);
+    if (parts) {
+      pmc[q] = mc;
+      pmx[q] = sx;
+    } else {
+      delta[t] = mc - sx;
+    }
+  }
+  __syncthreads();
+  for (int t = tid; parts && t < d; t += LB_T) {
     double mc = 0.0, mx = 0.0;
-    for (int k = 0; k < s; ++k) mc += aw[k] * A[(size_t)k * d + t];
-    for (int j = 0; j < m; ++j) mx += w[off + j] * supp[(size_t)(off + j) * d + t];
+    for (int part = 0; part < PD; ++part) {
+      mc += pmc[part * d + t];
+      mx += pmx[part * d + t];
+    }
     delta[t] = mc - mx;
   }
   __syncthreads();
@@ -475,7 +509,7 @@ int launch_group(d2mi_server *S, SrvGroup &G) {
   D2MI_HIP(hipMemcpyAsync(base, h, L.in_end, hipMemcpyHostToDevice, st));
   k_srv_zero_status<<<1, 64, 0, st>>>((int *)(base + L.status),
                                       (unsigned long long *)(base + L.nlp));
-  const size_t lds = sizeof(double) * ((size_t)s * d + 4 * LB_KC + d);
+  const size_t lds = sizeof(double) * ((size_t)s * d + 4 * LB_KC + d + 2 * LB_T);
   k_srv_bounds<<<R * K, LB_T, lds, st>>>(
       K, s, d, P.cs64.as<double>(), P.cw64.as<double>(), (const int *)(base + L.p_str),
       (const long long *)(base + L.cum), (const double *)(base + L.w),
