@@ -24,9 +24,21 @@ FLAGS = [
 ]
 
 
-def build(verbose: bool = False, jobs: int = 4, out: str = LIB, extra=()) -> str:
-    """Compiles every source and links ``out``; ``extra`` hipcc flags (for
-    example -D defines) apply to every translation unit."""
+def _stale(src: str, obj: str) -> bool:
+    """True when ``obj`` is older than its source or any header it may use."""
+    if not os.path.exists(obj):
+        return True
+    t = os.path.getmtime(obj)
+    deps = [src] + [os.path.join(CSRC, h) for h in os.listdir(CSRC) if h.endswith(".h")]
+    deps += [os.path.join(ROOT, "include", h) for h in os.listdir(os.path.join(ROOT, "include"))]
+    return any(os.path.getmtime(d) > t for d in deps)
+
+
+def build(verbose: bool = False, jobs: int = 4, out: str = LIB, extra=(),
+          changed_only: bool = False) -> str:
+    """Compiles every source (only out-of-date ones with ``changed_only``)
+    and links ``out``; ``extra`` hipcc flags (for example -D defines) apply
+    to every translation unit."""
     objdir = os.path.join(PKG, "_obj" if out == LIB else "_obj_" + os.path.basename(out))
     os.makedirs(objdir, exist_ok=True)
     procs = []
@@ -34,6 +46,8 @@ def build(verbose: bool = False, jobs: int = 4, out: str = LIB, extra=()) -> str
     for src in SOURCES:
         obj = os.path.join(objdir, src.replace(".hip", ".o"))
         objs.append(obj)
+        if changed_only and not _stale(os.path.join(CSRC, src), obj):
+            continue
         cmd = [HIPCC, *FLAGS, *extra, "-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
@@ -42,7 +56,11 @@ def build(verbose: bool = False, jobs: int = 4, out: str = LIB, extra=()) -> str
             _wait(procs)
     _wait(procs)
     tmp = out + ".tmp"
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+    # -Bsymbolic-functions: the library's own calls (d2_write from
+    # d2_clustering, ...) bind to its own definitions even in a process that
+    # also loaded another libad2c-like library globally
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-Wl,-Bsymbolic-functions",
+           "-o", tmp, *objs]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.check_call(cmd)
@@ -82,4 +100,4 @@ def _wait(procs):
 
 
 if __name__ == "__main__":
-    print(build(verbose=True))
+    print(build(verbose=True, changed_only="--changed" in sys.argv))

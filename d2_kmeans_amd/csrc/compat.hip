@@ -407,24 +407,145 @@ int d2_write_labels(const char *filename, mph *p) {
 }
 
 int d2_write_labels_serial(const char *filename_ind, const char *filename, mph *p) {
-  // clustering_io.c:277-324: with a shard index file, write labels back in
-  // the original object order as <filename>.label_o
+  // clustering_io.c:277-324: the labels of <filename>.label (as written by
+  // d2_write_labels, in shard order) are put back in the original object
+  // order given by the shard index file and written as <filename>.label_o;
+  // without an index file nothing is written
+  if (world_rank != 0) return 0;
+  const size_t n = p->global_size;
+  const std::string lf = std::string(filename) + ".label";
+  FILE *fl = fopen(lf.c_str(), "r");
+  if (!fl) {
+    fprintf(stderr, "d2_write_labels_serial: cannot open %s\n", lf.c_str());
+    return -1;
+  }
+  std::vector<int> lab(n, -1);
+  for (size_t i = 0; i < n; ++i)
+    if (fscanf(fl, "%d", &lab[i]) != 1) lab[i] = -1;
+  fclose(fl);
   FILE *fi = fopen(filename_ind, "r");
   if (!fi) return 0;
-  const size_t n = p->global_size;
   std::vector<size_t> ind(n);
   for (size_t i = 0; i < n; ++i)
     if (fscanf(fi, "%zu", &ind[i]) != 1) ind[i] = i;
   fclose(fi);
-  std::vector<int> lab(n, -1);
+  std::vector<int> out_lab(n, -1);
   for (size_t i = 0; i < n; ++i)
-    if (ind[i] < n) lab[ind[i]] = p->label[i];
+    if (ind[i] < n) out_lab[ind[i]] = lab[i];
   std::string out = std::string(filename) + ".label_o";
   FILE *fo = fopen(out.c_str(), "w");
   if (!fo) return -1;
-  for (size_t i = 0; i < n; ++i) fprintf(fo, "%d\n", lab[i]);
+  for (size_t i = 0; i < n; ++i) fprintf(fo, "%d\n", out_lab[i]);
   fclose(fo);
   printf("Write serial data labels to %s\n", out.c_str());
+  return 0;
+}
+
+int d2_write_split(const char *filename, mph *p, int splits, char is_pre_processed) {
+  // clustering_io.c:329-480: shuffle the objects (the serial glibc sequence
+  // of math.c:39-56), write the permutation to <filename>.ind and the
+  // objects in that order to <filename>.0 ... <filename>.<splits-1>, as
+  // they are or pre-processed (word embeddings become Euclidean supports,
+  // supports beyond the phase's stride are merged down to it, dense
+  // histograms become sparse ones)
+  if (!filename || splits <= 0) return -1;
+  const size_t size = p->size;
+  std::vector<size_t> idx(size);
+  for (size_t n = 0; n < size; ++n) idx[n] = n;
+  shuffle(idx.data(), size);
+  const std::string base(filename);
+  FILE *fp = fopen((base + ".ind").c_str(), "w+");
+  if (!fp) return -1;
+  for (size_t n = 0; n < size; ++n) fprintf(fp, "%zd\n", idx[n]);
+  fclose(fp);
+  fprintf(stderr, "\twrite %zd indices to %s.ind\n", size, filename);
+  const size_t batch = size ? 1 + (size - 1) / splits : 0;
+  printf("batch_size: %zd\n", batch);
+  std::vector<double> supp, w, csupp, cw;
+  std::vector<int> sym;
+  for (int k = 0; k < splits; ++k) {
+    const std::string fn = base + "." + std::to_string(k);
+    FILE *fo = fopen(fn.c_str(), "w+");
+    if (!fo) return -1;
+    size_t q = batch * k;
+    for (; q < size && q < batch * (k + 1); ++q) {
+      const size_t i = idx[q];
+      for (int n = 0; n < p->s_ph; ++n) {
+        const sph &s = p->ph[n];
+        if (s.col == 0) continue;
+        const int dim = s.dim;
+        int str = s.p_str[i];
+        const size_t pos = s.p_str_cum[i];
+        const double *pw = s.p_w + pos;
+        if (!is_pre_processed) {
+          if (s.metric_type == D2_HISTOGRAM) fprintf(fo, "%d %d ", dim, str);
+          else fprintf(fo, "%d\n%d\n", dim, str);
+          for (int j = 0; j < str; ++j) fprintf(fo, "%lf ", pw[j]);
+          fprintf(fo, "\n");
+          if (s.metric_type == D2_EUCLIDEAN_L2) {
+            for (int j = 0; j < str; ++j) {
+              for (int d = 0; d < dim; ++d) fprintf(fo, "%lf ", s.p_supp[(pos + j) * dim + d]);
+              fprintf(fo, "\n");
+            }
+          } else if (s.metric_type == D2_WORD_EMBED || s.metric_type == D2_SPARSE_HISTOGRAM) {
+            for (int j = 0; j < str; ++j) fprintf(fo, "%d ", s.p_supp_sym[pos + j] + 1);
+            fprintf(fo, "\n");
+          }
+          continue;
+        }
+        // pre-processed (:386-470)
+        const bool points = s.metric_type == D2_EUCLIDEAN_L2 || s.metric_type == D2_WORD_EMBED;
+        if (points) {
+          supp.resize((size_t)str * dim);
+          w.assign(pw, pw + str);
+          for (int j = 0; j < str; ++j)
+            memcpy(supp.data() + (size_t)j * dim,
+                   s.metric_type == D2_EUCLIDEAN_L2
+                       ? s.p_supp + (pos + j) * dim
+                       : s.vocab_vec + (size_t)s.p_supp_sym[pos + j] * dim,
+                   sizeof(double) * dim);
+          if (str > s.str) {
+            csupp.assign((size_t)s.str * dim, 0.0);
+            cw.assign(s.str, 0.0);
+            merge(dim, supp.data(), w.data(), str, csupp.data(), cw.data(), s.str);
+            supp.swap(csupp);
+            w.swap(cw);
+            str = s.str;
+          }
+        } else if (s.metric_type == D2_HISTOGRAM) {
+          sym.clear();
+          w.clear();
+          for (int j = 0; j < str; ++j)
+            if (pw[j] > 0) {
+              sym.push_back(j);
+              w.push_back(pw[j]);
+            }
+          str = (int)sym.size();
+        } else {
+          // a sparse histogram is already in the form pre-processing makes
+          // of a dense one: its weights and bin ids go out as they are (the
+          // reference leaves its weight pointer NULL in this case and
+          // crashes writing it)
+          w.assign(pw, pw + str);
+          sym.assign(s.p_supp_sym + pos, s.p_supp_sym + pos + str);
+        }
+        fprintf(fo, "%d\n%d\n", dim, str);
+        for (int j = 0; j < str; ++j) fprintf(fo, "%lf ", w[j]);
+        fprintf(fo, "\n");
+        if (points) {
+          for (int j = 0; j < str; ++j) {
+            for (int d = 0; d < dim; ++d) fprintf(fo, "%lf ", supp[(size_t)j * dim + d]);
+            fprintf(fo, "\n");
+          }
+        } else {
+          for (int j = 0; j < str; ++j) fprintf(fo, "%d ", sym[j] + 1);
+          fprintf(fo, "\n");
+        }
+      }
+    }
+    fclose(fo);
+    fprintf(stderr, "\twrite %zd objects to %s\n", q - batch * k, fn.c_str());
+  }
   return 0;
 }
 
@@ -657,9 +778,39 @@ double d2_compute_distance(mph *a, size_t i, mph *b, size_t j, int selected_phas
         _dpdist2(A->dim, nb, ma, Bp->p_supp + Bp->p_str_cum[j] * A->dim,
                  A->p_supp + A->p_str_cum[i] * A->dim, C.data());
         break;
+      case D2_WORD_EMBED: {
+        // _dpdist2_sym (blas_like64.c:210-220): the object's words are
+        // gathered from the vocabulary, a negative id standing for the zero
+        // vector, and the costs come from the device pdist2
+        const int d = A->dim;
+        std::vector<double> X((size_t)ma * d, 0.0);
+        for (int q = 0; q < ma; ++q) {
+          const int id = A->p_supp_sym[A->p_str_cum[i] + q];
+          if (id >= 0)
+            memcpy(X.data() + (size_t)q * d, A->vocab_vec + (size_t)id * d, sizeof(double) * d);
+        }
+        _dpdist2(d, nb, ma, Bp->p_supp + Bp->p_str_cum[j] * d, X.data(), C.data());
+        break;
+      }
       case D2_HISTOGRAM:
         memcpy(C.data(), A->dist_mat, sizeof(double) * nb * ma);
         break;
+      case D2_SPARSE_HISTOGRAM: {
+        // _dpdist2_submat (blas_like64.c:222-230, precomputed at
+        // clustering_util.c:189-199): column q is the ground-cost row of the
+        // object's bin, row k the centroid's bin k (a centroid holds all V
+        // bins).  A side with fewer than V bins is a sparse object and
+        // names its bins by id, so two objects can be compared as well.
+        const int V = A->vocab_size;
+        auto bins = [V](const sph *s, size_t o, int cnt, int q) {
+          return cnt < V ? s->p_supp_sym[s->p_str_cum[o] + q] : q;
+        };
+        for (int q = 0; q < ma; ++q) {
+          const size_t row = (size_t)bins(A, i, ma, q) * V;
+          for (int k = 0; k < nb; ++k) C[(size_t)q * nb + k] = A->dist_mat[row + bins(Bp, j, nb, k)];
+        }
+        break;
+      }
       default:
         fprintf(stderr, "d2_compute_distance: metric %d is not handled here\n", A->metric_type);
         return NAN;

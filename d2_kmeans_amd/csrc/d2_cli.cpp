@@ -1,8 +1,8 @@
 // d2 command line on the MI355X engine: the reference's flags and modes
 // (src/app/main.cc:53-282, src/app/README.md) over the reference-compatible
 // ABI of libd2mi.so.  Modes: clustering (default), --load <c.d2> (start from
-// given centroids), --eval <c.d2> (assign only).  --prepare_batches (MPI
-// shards) is not part of the single-GPU build.
+// given centroids), --eval <c.d2> (assign only), --prepare_batches <k> [-Q]
+// (write shards and their index file, no device work).
 #include <getopt.h>
 
 #include <cassert>
@@ -29,7 +29,7 @@ int main(int argc, char *argv[]) {
   long size_of_samples = 0;
   char *ss1 = nullptr, *ss2 = nullptr, *ss3 = nullptr, *filename = nullptr,
        *centroid_filename = nullptr, *meta_filename = nullptr, *output_filename = nullptr;
-  bool is_eval = false, is_load = false;
+  bool is_eval = false, is_load = false, is_pre_processed = false;
   char use_triangle = 1;
   int selected_phase = -1, number_of_clusters = 3, max_iters = 100, num_of_batches = 0;
   static struct option long_options[] = {
@@ -58,7 +58,7 @@ int main(int argc, char *argv[]) {
       case 'm': max_iters = atoi(optarg); break;
       case 'M': d2_alg_type = atoi(optarg); break;
       case 'T': use_triangle = 0; break;
-      case 'Q': break;
+      case 'Q': is_pre_processed = true; break;
       case 'P': num_of_batches = atoi(optarg); break;
       default:
         fprintf(stderr, "?? getopt returned character code 0%o ??\n", ch);
@@ -70,10 +70,6 @@ int main(int argc, char *argv[]) {
     fprintf(stderr, "usage: %s -i <file.d2> -n <objects> -d <dims> -s <strides> [-p phases] "
             "[-E types] [-k K] [-m max_iters] [-t phase] [-T] [-o out] [--load c.d2 | --eval c.d2]\n",
             argv[0]);
-    return 1;
-  }
-  if (num_of_batches > 0) {
-    fprintf(stderr, "--prepare_batches shards data for MPI ranks; this build runs on one GPU\n");
     return 1;
   }
   if (d2_alg_type != D2_CENTROID_BADMM) {
@@ -102,6 +98,13 @@ int main(int argc, char *argv[]) {
       d2_read(filename, meta_filename, &data)) {
     std::cerr << "Reading " << filename << " failed" << std::endl;
     return 1;
+  }
+  if (num_of_batches == 0 && is_pre_processed) num_of_batches = 1;  // main.cc:181
+  if (num_of_batches > 0) {  // main.cc:185-194
+    const int rc = d2_write_split(filename, &data, num_of_batches, is_pre_processed);
+    d2_free(&data);
+    std::cout << "[Finish!]" << std::endl;
+    return rc ? 1 : 0;
   }
   mph c;
   c.ph = nullptr;

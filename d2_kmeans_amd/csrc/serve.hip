@@ -736,11 +736,23 @@ extern "C" int d2mi_serve_submit(d2mi_server *S, int m, const double *w, const d
     d2mi_set_error("query of %d support points does not fit the server's memory", m);
     return D2MI_ENOMEM;
   }
+  // the bounds and the LP both assume unit mass on each side: weights are
+  // renormalised as d2_read does for every object it loads
+  // (clustering_io.c:143-151), and a weight <= 1e-9 is refused as there
+  double wsum = 0.0;
+  for (int j = 0; j < m; ++j) {
+    if (!(w[j] > 1e-9) || !std::isfinite(w[j])) {
+      d2mi_set_error("query weight %d is %g; every weight must be finite and > 1e-9", j, w[j]);
+      return D2MI_EINVAL;
+    }
+    wsum += w[j];
+  }
   const long long t = S->next_ticket++;
   SrvQuery &Q = S->q[t];
   Q.m = m;
   Q.cls = size_class(m);
-  Q.w.assign(w, w + m);
+  Q.w.resize(m);
+  for (int j = 0; j < m; ++j) Q.w[j] = w[j] / wsum;
   Q.supp.assign(supp, supp + (size_t)m * S->d);
   Q.t_submit = now_ms();
   S->queue.push_back(t);

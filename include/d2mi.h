@@ -242,7 +242,10 @@ typedef struct {
  * opts may be NULL.  max_batch = 1 with streams = 1 serves one query per
  * launch, in arrival order (the plain form). */
 int d2mi_serve_open(d2mi_ctx *ctx, int phase, const d2mi_serve_opts *opts, d2mi_server **out);
-/* Queues one query: m support points (supp: m x d row-major, w: m weights). */
+/* Queues one query: m support points (supp: m x d row-major, w: m weights).
+ * The weights are renormalised to sum 1, as d2_read does for every object
+ * (clustering_io.c:143-151); a weight that is not finite or is <= 1e-9 is
+ * refused with D2MI_EINVAL (the reference asserts there, :145). */
 int d2mi_serve_submit(d2mi_server *srv, int m, const double *w, const double *supp,
                       long long *ticket);
 /* Collects finished groups and admits waiting queries; *pending (may be

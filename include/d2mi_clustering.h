@@ -111,6 +111,9 @@ int d2_read(const char *filename, const char *meta_filename, mph *p_data);
 int d2_write(const char *filename, mph *p_data);
 int d2_write_labels(const char *filename, mph *p_data);
 int d2_write_labels_serial(const char *filename_ind, const char *filename, mph *p_data);
+/* shards for the reference's MPI runs (clustering.h:151, clustering_io.c:329-480):
+ * <filename>.ind and <filename>.0 ... <filename>.<splits-1> */
+int d2_write_split(const char *filename, mph *p_data, int splits, char is_pre_processed);
 int d2_free(mph *p_data);
 
 /* centroids (clustering.h:152-155, 216-218; clustering.c:379-404,

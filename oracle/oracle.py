@@ -43,7 +43,13 @@ def build_ref(force=False):
     if force:
         args.insert(2, "-B")
     subprocess.check_call(args, cwd=root)
+    # the reference's own program on the product library, when that is built
+    if os.path.exists(os.path.join(root, "d2_kmeans_amd", "libd2mi.so")):
+        subprocess.check_call(args + ["main"], cwd=root)
     return True
+
+
+REF_MAIN = os.path.join(HERE, "_ref", "d2_main_on_d2mi")
 
 
 def ref_available():
@@ -109,6 +115,7 @@ def lib():
         L.or_labeling_post.argtypes = [C.c_int, C.POINTER(OrPhase), C.c_int,
                                        C.POINTER(OrCentroids), C.POINTER(OrCentroids), C.c_int,
                                        _ip, C.POINTER(OrElkan)]
+        L.or_calculate_distmat.argtypes = [C.POINTER(OrPhase), _ip, C.POINTER(OrCentroids), _dp]
         L.or_shuffle.argtypes = [_sp, C.c_size_t, C.c_uint]
         L.or_merge.argtypes = [C.c_int, _dp, _dp, C.c_int, _dp, _dp, C.c_int]
         L.or_centroid_rands.argtypes = [C.POINTER(OrPhase), C.c_int, C.c_int,
@@ -197,6 +204,20 @@ def badmm(phase, label, label_switch, cent, Z, opts, max_trace=64):
                         C.byref(ntr), C.byref(obj))
     return dict(rc=rc, rho=rho.value, obj=obj.value,
                 trace=trace[:4 * min(ntr.value, max_trace)].reshape(-1, 4), X=X, Y=Y, C=Cb)
+
+
+def hist_cost(phase, s):
+    """The restatement's cost blocks of a histogram phase (or_calculate_distmat
+    with centroids of s = V bins; labels do not enter)."""
+    from d2_kmeans_amd.data import Centroids
+    keep = _Keep()
+    out = np.zeros(s * phase.col)
+    lab = keep.add(np.zeros(phase.size, np.int32))
+    c = Centroids(1, s, 0, keep.add(np.full(s, 1.0 / s)))
+    cs = or_cent(c)
+    lib().or_calculate_distmat(C.byref(or_phase(phase, keep)), ptr(lab, _ip), C.byref(cs),
+                               ptr(out))
+    return out
 
 
 def _phases(data, keep):
@@ -368,6 +389,7 @@ def ref():
         L.d2_centroid_sphBregman.argtypes = [C.POINTER(RefMph), C.POINTER(RefVarMph), C.c_int,
                                              C.POINTER(RefSph), C.POINTER(RefSph)]
         L._dpdist2.argtypes = [C.c_int, C.c_size_t, C.c_size_t, _dp, _dp, _dp]
+        L._dpdist2_submat.argtypes = [C.c_size_t, _ip, _dp, C.c_int, _dp]
         L.merge.argtypes = [C.c_int, _dp, _dp, C.c_int, _dp, _dp, C.c_int]
         L.shuffle.argtypes = [_sp, C.c_size_t]
         L.d2_centroid_rands.argtypes = [C.POINTER(RefMph), C.c_int, C.POINTER(RefSph)]
@@ -418,9 +440,35 @@ def ref_set_badmm_options(opts):
     C.c_double.in_dll(L, "time_budget").value = 1e30  # never break on wall clock
 
 
+def ref_prefill_hist_cost(phase, s, Cb):
+    """The ground-cost blocks the reference precomputes once for histogram
+    phases, because its calculate_distmat skips them (centroid_util.c:26-28):
+    clustering_util.c:183-199.  Dense histograms copy the phase's V x V
+    ``dist_mat`` into every object's block (the reference indexes ``ph[k]``
+    by object there, a bug; the phase's own matrix is what it means); sparse
+    histograms take the rows of their bins through the reference's own
+    ``_dpdist2_submat`` (blas_like64.c:222-230)."""
+    L = ref()
+    cum = phase.p_str_cum
+    if phase.metric == 5:  # D2_HISTOGRAM
+        for k in range(phase.size):
+            n = s * int(phase.p_str[k])
+            Cb[s * int(cum[k]):s * int(cum[k]) + n] = phase.dist_mat[:n]
+    elif phase.metric == 12:  # D2_SPARSE_HISTOGRAM
+        dm = np.ascontiguousarray(phase.dist_mat, np.float64)
+        sym = np.ascontiguousarray(phase.p_supp_sym, np.int32)
+        for k in range(phase.size):
+            off = int(cum[k])
+            L._dpdist2_submat(int(phase.p_str[k]), sym[off:].ctypes.data_as(_ip),
+                              Cb[s * off:].ctypes.data_as(_dp), int(phase.vocab_size), ptr(dm))
+
+
 def ref_badmm(phase, label, label_switch, cent, Z, opts):
     """Runs the reference's d2_centroid_sphBregman (centroid_Bregman.c:51)
-    on one phase, in place on ``cent`` and ``Z``; returns (X, Y, C)."""
+    on one phase, in place on ``cent`` and ``Z``; returns (X, Y, C).
+    Histogram phases get their cost blocks pre-filled first, as the
+    reference's work allocation does (``ref_prefill_hist_cost``); a fresh
+    block per call, so every call starts from the raw ground cost."""
     L = ref()
     ref_set_badmm_options(opts)
     keep = _Keep()
@@ -432,6 +480,7 @@ def ref_badmm(phase, label, label_switch, cent, Z, opts):
     s = cent.s
     sc = s * phase.col
     Cb = np.zeros(s * (phase.col + cent.K * s))
+    ref_prefill_hist_cost(phase, s, Cb)
     X = np.zeros(sc)
     Y = np.zeros(sc)
     Xc = np.zeros(phase.col)

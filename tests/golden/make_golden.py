@@ -6,6 +6,10 @@
                      oracle/Makefile.ref) on a seeded 3-d Gaussian-mixture
                      case: a cold call (every label switched) and a warm
                      call (a third of the labels switched), maxIters = 10.
+* ref_badmm_hist.npz -- the same two calls on histogram phases: a dense
+                     16-bin and a sparse 64-bin phase with an asymmetric
+                     ground cost, the reference's cost blocks pre-filled as
+                     clustering_util.c:183-199 intends (oracle.ref_badmm).
 * ref_pdist2.npz  -- the reference's _dpdist2 (blas_like64.c:199-208).
 * ref_rands.npz   -- the reference's d2_centroid_rands (centroid_rand.c:161)
                      with its glibc-rand shuffle (math.c:39-56), K=8, s=6.
@@ -13,6 +17,15 @@
                      poses to Mosek (solver_mosek.cc:61-143: row sums = wX,
                      column sums = wY except the last), solved by scipy's
                      HiGHS, an independent exact LP solver.
+* lp_highs_large.npz -- the same LP at the serving shapes (48 x 200 ... 48 x
+                     500, 64 x 500), stored as support points (d = 8) and
+                     weights; the cost is the oracle's pdist2 of them.
+* oracle_c2_loop.npz -- the FP64 oracle's closed clustering loop (Elkan
+                     labeling, B-ADMM maxIters = 10, 5 outer iterations) on
+                     the image-like workload at N = 3000 (16 x 16 grid
+                     strokes, K = 10, s = 40): label changes per iteration,
+                     final labels, centroids and cost.  The input is
+                     regenerated from its seed and checked by hash.
 * mountaindat_head.d2 -- the first 300 objects of the reference's
                      data/mountaindat.d2 (2 phases, d = 3), an input file.
 
@@ -61,6 +74,33 @@ def make_badmm():
              c0_w=c0.p_w, c0_supp=c0.p_supp, label=label, lsw=lsw,
              c1_w=c1.p_w, c1_supp=c1.p_supp, Z1=Z1, X1=X1, Y1=Y1,
              label2=label2, lsw2=lsw2, c2_w=c2.p_w, c2_supp=c2.p_supp, Z2=Z, X2=X2, Y2=Y2)
+
+
+def make_badmm_hist():
+    sys.path.insert(0, os.path.dirname(HERE))
+    from cases import hist_phase, hist_state
+    opts = BADMMOptions(10, 2.0, 10)
+    out = dict(maxIters=opts.maxIters, rhoCoeff=opts.rhoCoeff, updatePerLoops=opts.updatePerLoops)
+    for kind, n, V, K in (("dense", 40, 16, 3), ("sparse", 30, 64, 3)):
+        ph = hist_phase(kind, n, V, seed=V + n)
+        c0, label = hist_state(ph, K, seed=5)
+        lsw = np.ones(n, np.int8)
+        c = c0.copy()
+        Z = np.zeros(V * ph.col)
+        O.ref_badmm(ph, label, lsw, c, Z, opts)
+        c1_w, Z1 = c.p_w.copy(), Z.copy()
+        rng = np.random.default_rng(9)
+        label2 = label.copy()
+        mv = rng.random(n) < 1 / 3
+        label2[mv] = (label2[mv] + 1) % K
+        X2, _, _ = O.ref_badmm(ph, label2, mv.astype(np.int8), c, Z, opts)
+        rec = dict(V=V, K=K, p_str=ph.p_str, p_w=ph.p_w, dist_mat=ph.dist_mat, c0_w=c0.p_w,
+                   label=label, lsw=lsw, c1_w=c1_w, Z1=Z1, label2=label2,
+                   lsw2=mv.astype(np.int8), c2_w=c.p_w, Z2=Z, X2=X2)
+        if kind == "sparse":
+            rec["sym"] = ph.p_supp_sym
+        out.update({f"{kind}_{k}": v for k, v in rec.items()})
+    np.savez_compressed(os.path.join(HERE, "ref_badmm_hist.npz"), **out)
 
 
 def make_pdist2():
@@ -122,6 +162,40 @@ def make_lp():
              wY=np.concatenate(out["wY"]), val=np.array(out["val"]))
 
 
+def make_lp_large():
+    rng = np.random.default_rng(17)
+    shapes = [(48, 200), (48, 300), (48, 500), (64, 500), (32, 400)]
+    out = dict(n=[], m=[], A=[], B=[], wX=[], wY=[], val=[])
+    for n, m in shapes:
+        A = rng.standard_normal((n, 8))
+        B = rng.standard_normal((m, 8)) + 0.3
+        Cm = O.pdist2(A, B)  # C[j*n + i] = |A_i - B_j|^2, element (i, j) at [i + j*n]
+        wX = rng.dirichlet(np.ones(n))
+        wY = rng.dirichlet(np.ones(m))
+        out["n"].append(n)
+        out["m"].append(m)
+        out["A"].append(A.reshape(-1))
+        out["B"].append(B.reshape(-1))
+        out["wX"].append(wX)
+        out["wY"].append(wY)
+        out["val"].append(lp_highs(Cm, wX, wY))
+    np.savez_compressed(os.path.join(HERE, "lp_highs_large.npz"), n=np.array(out["n"]),
+                        m=np.array(out["m"]), val=np.array(out["val"]),
+                        **{k: np.concatenate(out[k]) for k in ("A", "B", "wX", "wY")})
+
+
+def make_c2_loop():
+    sys.path.insert(0, os.path.dirname(HERE))
+    from cases import c2_loop_case
+    data, c0, digest = c2_loop_case()
+    c = c0.copy()
+    lab, it, trace, cost = O.clustering(data, [c], 10, 5, BADMMOptions(10, 2.0, 10),
+                                        use_triangle=True)
+    np.savez_compressed(os.path.join(HERE, "oracle_c2_loop.npz"), digest=np.array(digest),
+                        iterations=it, changes=trace, label=lab.astype(np.int8), c_w=c.p_w,
+                        c_supp=c.p_supp, cost=cost)
+
+
 def make_mountaindat():
     src = "/root/reference/data/mountaindat.d2"
     lines = open(src).read().split("\n")
@@ -141,8 +215,11 @@ if __name__ == "__main__":
     O.build()
     assert O.build_ref(), "needs /root/reference"
     make_badmm()
+    make_badmm_hist()
     make_pdist2()
     make_rands()
     make_lp()
+    make_lp_large()
+    make_c2_loop()
     make_mountaindat()
     print("golden fixtures written to", HERE)

@@ -131,3 +131,36 @@ def test_clustering_loop_vs_oracle(engine, use_triangle):
             assert abs(st["final_cost"] - cost_o) <= 1e-9 * cost_o
         else:
             assert abs(st["final_cost"] - cost_o) <= 1e-5 * cost_o
+
+
+def test_image_like_clustering_loop_vs_oracle_golden(engine):
+    """The closed clustering loop (clustering.c:409-525: Elkan labeling with
+    exact LPs, B-ADMM, label switching, bound relaxation) on the image-like
+    workload at N = 3000 (16 x 16 grid strokes, m in [20, 80], K = 10,
+    s = 40, 5 outer iterations) in FP64: the label changes of every outer
+    iteration, the final labels, centroids and cost equal the FP64 oracle's
+    (tests/golden/oracle_c2_loop.npz; the oracle needs minutes for this
+    loop, so its trajectory is a fixture).  The FP32 run keeps the final
+    labels and the cost within 1e-5."""
+    from cases import c2_loop_case
+    g = np.load(os.path.join(GOLD, "oracle_c2_loop.npz"), allow_pickle=False)
+    data, c0, digest = c2_loop_case()
+    assert digest == str(g["digest"]), "the generator no longer makes the fixture's input"
+    say("image-like clustering loop, N=3000, K=10, s=40")
+    opts = BADMMOptions(10, 2.0, 10)
+    for precision in (64, 32):
+        setup(engine, data, [c0])
+        st = engine.clustering(10, 5, opts, use_triangle=True, precision=precision)
+        lab, _ = engine.get_labels()
+        say(f"fp{precision}: changes {st['changes']} oracle {list(g['changes'])}, "
+            f"labeling {st['ms_labeling']:.1f} ms, B-ADMM {st['ms_badmm']:.1f} ms")
+        np.testing.assert_array_equal(lab, g["label"].astype(np.int32))
+        if precision == 64:
+            assert st["iterations"] == int(g["iterations"])
+            assert st["changes"] == [int(x) for x in g["changes"]]
+            cg = engine.get_centroids(0)
+            np.testing.assert_allclose(cg.p_supp, g["c_supp"], rtol=1e-8, atol=1e-9)
+            np.testing.assert_allclose(cg.p_w, g["c_w"], rtol=1e-8, atol=1e-12)
+            assert abs(st["final_cost"] - float(g["cost"])) <= 1e-9 * float(g["cost"])
+        else:
+            assert abs(st["final_cost"] - float(g["cost"])) <= 1e-5 * float(g["cost"])

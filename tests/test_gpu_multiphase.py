@@ -4,14 +4,18 @@
 histograms, clustering_io.c:182-185), against the FP64 oracle.  The sparse
 phase's plans (1024 x m) exceed LDS, so its B-ADMM runs the HBM-scratch plan
 kernel."""
+import os
+
 import numpy as np
 import pytest
 
 from d2_kmeans_amd import synth
 from d2_kmeans_amd.data import BADMMOptions, Centroids, D2Data
+from cases import hist_golden_case, hist_phase, hist_state
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
 
 
 def say(*a):
@@ -55,6 +59,49 @@ def test_histogram_phase_badmm_vs_oracle(engine, c4, which):
         cg = engine.get_centroids(0)
         tol = 1e-9 if precision == 64 else 1e-4
         assert rel(cg.p_w, co.p_w) < tol
+        assert abs(st["obj"] - ro["obj"]) <= (1e-9 if precision == 64 else 1e-5) * abs(ro["obj"])
+
+
+@pytest.mark.parametrize("kind", ["dense", "sparse"])
+def test_histogram_badmm_vs_reference_golden(engine, kind):
+    """The reference build's own histogram B-ADMM (golden vectors, an
+    asymmetric ground cost so a transposed cost layout fails): a cold call,
+    then a warm call with a third of the labels switched and the plans kept
+    on the device."""
+    g = np.load(os.path.join(GOLD, "ref_badmm_hist.npz"), allow_pickle=False)
+    ph, c0, label, lsw, label2, lsw2, opts = hist_golden_case(g, kind)
+    say(f"histogram golden {kind}: V={ph.vocab_size}, N={ph.size}")
+    for precision, tol, ztol in ((64, 1e-9, 1e-8), (32, 1e-4, 2e-3)):
+        engine.set_data(D2Data([ph]))
+        engine.set_centroids(0, c0)
+        engine.set_labels(label, lsw)
+        engine.badmm(0, opts, precision=precision)
+        assert rel(engine.get_centroids(0).p_w, g[kind + "_c1_w"]) < tol
+        assert rel(engine.get_plan(0, precision), g[kind + "_Z1"]) < ztol
+        engine.set_labels(label2, lsw2)
+        engine.badmm(0, opts, precision=precision)
+        assert rel(engine.get_centroids(0).p_w, g[kind + "_c2_w"]) < tol
+        assert rel(engine.get_plan(0, precision), g[kind + "_Z2"]) < ztol
+
+
+@pytest.mark.parametrize("kind,n,V,K", [("dense", 300, 64, 5), ("sparse", 150, 1024, 4)])
+def test_histogram_badmm_asymmetric_cost_vs_oracle(engine, kind, n, V, K):
+    """Dense 64-bin and sparse 1024-bin phases with an asymmetric ground
+    cost (the oracle is pinned on these by the reference build)."""
+    ph = hist_phase(kind, n, V, seed=V + n)
+    c0, label = hist_state(ph, K, seed=7)
+    lsw = np.ones(n, np.int8)
+    opts = BADMMOptions(10, 2.0, 10)
+    co = c0.copy()
+    ro = O.badmm(ph, label, lsw, co, np.zeros(V * ph.col), opts)
+    say(f"asymmetric {kind}: V={V}, N={n}, oracle obj {ro['obj']:.10g}")
+    for precision in (64, 32):
+        engine.set_data(D2Data([ph]))
+        engine.set_centroids(0, c0)
+        engine.set_labels(label, lsw)
+        st = engine.badmm(0, opts, precision=precision)
+        tol = 1e-9 if precision == 64 else 1e-4
+        assert rel(engine.get_centroids(0).p_w, co.p_w) < tol
         assert abs(st["obj"] - ro["obj"]) <= (1e-9 if precision == 64 else 1e-5) * abs(ro["obj"])
 
 

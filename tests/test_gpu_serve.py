@@ -37,6 +37,31 @@ def oracle_answers(qph, cent):
     return lab, dist
 
 
+def oracle_answers_threads(qph, cent, workers=8):
+    """The same brute force, one query per thread: every centroid's exact LP
+    (the oracle's simplex releases the interpreter lock while it solves),
+    first minimum on ties (clustering.c:344)."""
+    from concurrent.futures import ThreadPoolExecutor
+    O.lib()
+    K, s, d = cent.K, cent.s, qph.dim
+    CS = cent.p_supp.reshape(K, s, d)
+    CW = cent.p_w.reshape(K, s)
+
+    def one(i):
+        w, X = qph.object(i)
+        best, lab = None, -1
+        for k in range(K):
+            Cm = O.pdist2(CS[k], X)  # element (a, j) at [a + j*s]
+            v = np.sqrt(max(O.transport(Cm, CW[k], w), 0.0))
+            if best is None or v < best:
+                best, lab = v, k
+        return lab, best
+
+    with ThreadPoolExecutor(workers) as ex:
+        res = list(ex.map(one, range(qph.size)))
+    return np.array([r[0] for r in res]), np.array([r[1] for r in res])
+
+
 @pytest.fixture(scope="module")
 def setup(engine):
     say("serve setup: d=64 training phase, K=12 centroids of 16 points")
@@ -133,3 +158,59 @@ def test_oversized_query_is_rejected(setup):
     with pytest.raises(D2MIError):
         srv.submit(np.full(101, 1 / 101), rng.standard_normal((101, srv.dim)))
     srv.close()
+
+
+def test_unnormalised_weights_are_renormalised(setup):
+    """Weights that do not sum to 1 are renormalised on submission, as
+    d2_read renormalises every object (clustering_io.c:143-151): labels and
+    distances equal the oracle's on the normalised query.  (The server's
+    lower bounds assume unit mass; without this a bound could exceed the
+    LP value and drop the true nearest centroid.)  A weight <= 1e-9 is
+    refused."""
+    eng, cent, qph, lab, dist = setup
+    say("unnormalised weights")
+    from d2_kmeans_amd._lib import D2MIError
+    rng = np.random.default_rng(8)
+    srv = Server(eng, 0, max_batch=16, streams=4)
+    qs = [(w * rng.choice([0.01, 0.3, 2.5, 40.0]), X) for w, X in queries_of(qph)]
+    got_l, got_d = srv.assign(qs)
+    np.testing.assert_array_equal(got_l, lab)
+    np.testing.assert_allclose(got_d, dist, rtol=1e-9, atol=1e-12)
+    w, X = qph.object(0)
+    bad = w.copy()
+    bad[0] = 0.0
+    with pytest.raises(D2MIError):
+        srv.submit(bad, X)
+    srv.close()
+
+
+def test_server_at_serving_shape(engine):
+    """The serving workload's shape: K = 64 centroids of 48 points in d = 64,
+    queries of lognormal(ln 50, 1) size clipped to [5, 500] plus four of the
+    maximum 500 points, against the oracle's brute force over all 64
+    centroids (exact LPs up to 48 x 500)."""
+    say("serving shape: K=64 x 48, d=64, m <= 500")
+    rng = np.random.default_rng(77)
+    means = rng.standard_normal((64, 64)) * 2.0
+    train, _ = synth.serving_requests(n=600, seed=201, means=means)
+    cent = synth.seed_centroids(train.phases[0], 64, 48)
+    engine.set_data(train)
+    engine.set_centroids(0, cent)
+    q1, _ = synth.serving_requests(n=72, seed=202, means=means)
+    q2, _ = synth.serving_requests(n=4, seed=203, m_lo=500, m_hi=500, mu=np.log(500.0),
+                                   sigma=0.01, means=means)
+    qs = queries_of(q1.phases[0]) + queries_of(q2.phases[0])
+    assert max(len(w) for w, _ in qs) == 500
+    from d2_kmeans_amd.data import Phase
+    qph = Phase(metric=0, dim=64, p_str=np.array([len(w) for w, _ in qs], np.int32),
+                p_w=np.concatenate([w for w, _ in qs]),
+                p_supp=np.concatenate([X.reshape(-1) for _, X in qs]), str=48)
+    lab, dist = oracle_answers_threads(qph, cent)
+    srv = Server(engine, 0, max_batch=16, streams=6)
+    got_l, got_d = srv.assign(qs)
+    st = srv.stats()
+    srv.close()
+    np.testing.assert_array_equal(got_l, lab)
+    np.testing.assert_allclose(got_d, dist, rtol=1e-9, atol=1e-12)
+    assert st["lps"] < len(qs) * 64
+    say("stats", st)

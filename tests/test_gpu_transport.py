@@ -28,12 +28,25 @@ def test_lp_golden_highs(engine):
         assert abs(v - val) <= 1e-9 * max(1.0, abs(val)), (n, m, v, val)
 
 
-@pytest.mark.parametrize("n,m,nb", [(6, 12, 64), (40, 80, 32), (64, 200, 8), (1, 7, 4), (9, 1, 4)])
+def test_lp_golden_highs_serving_shapes(engine):
+    """HiGHS values at the serving shapes (48 x 200 ... 48 x 500, 64 x 500),
+    one LP per launch and all of one shape in a batch."""
+    from cases import lp_large_cases
+    print("transport golden, serving shapes", flush=True)
+    for n, m, Cm, wX, wY, val in lp_large_cases(np.load(os.path.join(GOLD, "lp_highs_large.npz"),
+                                                        allow_pickle=False)):
+        v = engine.transport_batch(Cm[None], wX[None], wY[None])[0]
+        assert abs(v - val) <= 1e-9 * max(1.0, abs(val)), (n, m, v, val)
+
+
+@pytest.mark.parametrize("n,m,nb", [(6, 12, 64), (40, 80, 32), (64, 200, 8), (1, 7, 4), (9, 1, 4),
+                                    (48, 300, 6), (48, 500, 4)])
 def test_lp_batch_vs_oracle(engine, n, m, nb):
     print(f"transport batch n={n} m={m} nb={nb}", flush=True)
     rng = np.random.default_rng(n * 1000 + m)
-    A = rng.standard_normal((nb, n, 2))
-    B = rng.standard_normal((nb, m, 2))
+    d = 64 if m >= 300 else 2  # the serving workload's d = 64 at its largest LPs
+    A = rng.standard_normal((nb, n, d))
+    B = rng.standard_normal((nb, m, d))
     C = ((A[:, :, None, :] - B[:, None, :, :]) ** 2).sum(-1)  # (nb, n, m)
     Cf = np.ascontiguousarray(C.transpose(0, 2, 1)).reshape(nb, -1)  # [i + j*n]
     wX = rng.dirichlet(np.ones(n), size=nb)

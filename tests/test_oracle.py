@@ -11,6 +11,7 @@ import pytest
 from d2_kmeans_amd import synth
 from d2_kmeans_amd.data import (BADMMOptions, Centroids, D2Data, D2_EUCLIDEAN_L2, Phase,
                                 read_d2)
+from cases import hist_golden_case, hist_phase, hist_state
 from oracle import oracle as O
 
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
@@ -114,6 +115,76 @@ def test_badmm_stored_cost_phases_match_reference_live(kind):
 
 
 @needs_ref
+@pytest.mark.parametrize("kind,n,V,K", [("dense", 100, 64, 4), ("sparse", 60, 1024, 3)])
+def test_badmm_histogram_phases_match_reference_live(kind, n, V, K):
+    """Histogram B-ADMM (no support update, centroid_Bregman.c:265-267) of
+    the restatement against the reference build, whose cost blocks are
+    pre-filled as clustering_util.c:183-199 does: a cold call, then a warm
+    call with a third of the labels switched.  The same restatement run on
+    the transposed ground cost must NOT agree, which shows the check sees
+    the cost layout."""
+    ph = hist_phase(kind, n, V, seed=V + n)
+    c0, label = hist_state(ph, K, seed=5)
+    opts = BADMMOptions(10, 2.0, 10)
+    lsw = np.ones(n, np.int8)
+    ca, cb = c0.copy(), c0.copy()
+    Za, Zb = np.zeros(V * ph.col), np.zeros(V * ph.col)
+    ra = O.badmm(ph, label, lsw, ca, Za, opts)
+    Xb, _, _ = O.ref_badmm(ph, label, lsw, cb, Zb, opts)
+    assert ra["rc"] == 0
+    np.testing.assert_allclose(ca.p_w, cb.p_w, rtol=1e-10, atol=1e-14)
+    np.testing.assert_allclose(Za, Zb, rtol=1e-9, atol=1e-15)
+    np.testing.assert_allclose(ra["X"], Xb, rtol=1e-9, atol=1e-15)
+    rng = np.random.default_rng(9)
+    label2 = label.copy()
+    mv = rng.random(n) < 1 / 3
+    label2[mv] = (label2[mv] + 1) % K
+    O.badmm(ph, label2, mv.astype(np.int8), ca, Za, opts)
+    O.ref_badmm(ph, label2, mv.astype(np.int8), cb, Zb, opts)
+    np.testing.assert_allclose(ca.p_w, cb.p_w, rtol=1e-10, atol=1e-14)
+    np.testing.assert_allclose(Za, Zb, rtol=1e-9, atol=1e-15)
+    # the transposed cost gives other centroids
+    pt = Phase(metric=ph.metric, dim=0, p_str=ph.p_str, p_w=ph.p_w, p_supp_sym=ph.p_supp_sym,
+               vocab_size=V, dist_mat=ph.dist_mat.reshape(V, V).T.copy().reshape(-1), str=V)
+    ct = c0.copy()
+    O.badmm(pt, label, lsw, ct, np.zeros(V * ph.col), opts)
+    c1 = c0.copy()
+    O.ref_badmm(ph, label, lsw, c1, np.zeros(V * ph.col), opts)
+    assert np.max(np.abs(ct.p_w - c1.p_w)) > 1e-4 * np.max(c1.p_w)
+
+
+@needs_ref
+def test_pdist2_submat_matches_reference_live():
+    """or_pdist2_submat (the sparse-histogram cost gather) against the
+    reference's _dpdist2_submat, bit for bit."""
+    ph = hist_phase("sparse", 20, 50, seed=3)
+    s = 50
+    Ca = np.zeros(s * ph.col)
+    O.ref_prefill_hist_cost(ph, s, Ca)
+    Cb = O.hist_cost(ph, s)
+    np.testing.assert_array_equal(Ca, Cb)
+
+
+def test_badmm_histograms_match_reference_golden():
+    """The reference build's histogram B-ADMM (cost blocks pre-filled as
+    clustering_util.c:183-199 intends), kept as golden vectors: a dense
+    16-bin and a sparse 64-bin phase with an asymmetric ground cost, a cold
+    call then a warm call with a third of the labels switched."""
+    g = gold("ref_badmm_hist.npz")
+    for kind in ("dense", "sparse"):
+        ph, c, label, lsw, label2, lsw2, opts = hist_golden_case(g, kind)
+        Z = np.zeros(c.s * ph.col)
+        r1 = O.badmm(ph, label, lsw, c, Z, opts)
+        assert r1["rc"] == 0
+        np.testing.assert_allclose(c.p_w, g[kind + "_c1_w"], rtol=1e-10, atol=1e-14)
+        np.testing.assert_allclose(Z, g[kind + "_Z1"], rtol=1e-9, atol=1e-15)
+        r2 = O.badmm(ph, label2, lsw2, c, Z, opts)
+        np.testing.assert_allclose(c.p_w, g[kind + "_c2_w"], rtol=1e-10, atol=1e-14)
+        np.testing.assert_allclose(Z, g[kind + "_Z2"], rtol=1e-9, atol=1e-15)
+        np.testing.assert_allclose(r2["X"], g[kind + "_X2"], rtol=1e-9, atol=1e-15)
+
+
+@needs_ref
 def test_merge_matches_reference_live():
     rng = np.random.default_rng(0)
     for m, n in [(12, 6), (9, 2), (40, 39), (80, 40)]:
@@ -141,6 +212,14 @@ def test_lp_matches_highs():
         oc += n * m
         ox += n
         oy += m
+        v = O.transport(Cm, wX, wY)
+        assert abs(v - val) <= 1e-9 * max(1.0, abs(val)), (n, m, v, val)
+
+
+def test_lp_matches_highs_serving_shapes():
+    """The same at the serving shapes, 48 x 200 ... 48 x 500 and 64 x 500."""
+    from cases import lp_large_cases
+    for n, m, Cm, wX, wY, val in lp_large_cases(gold("lp_highs_large.npz")):
         v = O.transport(Cm, wX, wY)
         assert abs(v - val) <= 1e-9 * max(1.0, abs(val)), (n, m, v, val)
 
