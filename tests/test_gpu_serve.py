@@ -72,11 +72,20 @@ def setup(engine):
     qdata, _ = synth.serving_requests(n=60, seed=102, m_lo=5, m_hi=300)
     qph = qdata.phases[0]
     lab, dist = oracle_answers(qph, cent)
-    return engine, cent, qph, lab, dist
+    return engine, cent, qph, lab, dist, train
+
+
+def _resident(setup):
+    """Puts the fixture's training set and centroids back on the device
+    (other tests upload their own)."""
+    eng, cent, qph, lab, dist, train = setup
+    eng.set_data(train)
+    eng.set_centroids(0, cent)
+    return eng, cent, qph, lab, dist
 
 
 def test_server_matches_oracle(setup):
-    eng, cent, qph, lab, dist = setup
+    eng, cent, qph, lab, dist = _resident(setup)
     say("server vs oracle brute force")
     srv = Server(eng, 0, max_batch=16, streams=4)
     got_l, got_d = srv.assign(queries_of(qph))
@@ -91,7 +100,7 @@ def test_server_matches_oracle(setup):
 
 
 def test_alone_and_in_the_mix_agree(setup):
-    eng, cent, qph, lab, dist = setup
+    eng, cent, qph, lab, dist = _resident(setup)
     say("alone vs mix")
     plain = Server(eng, 0, plain=True)
     alone_l, alone_d = [], []
@@ -116,7 +125,7 @@ def test_alone_and_in_the_mix_agree(setup):
 
 
 def test_admission_waits_for_memory(setup):
-    eng, cent, qph, lab, dist = setup
+    eng, cent, qph, lab, dist = _resident(setup)
     say("small HBM budget")
     big = int(qph.p_str.max())
     per_query = big * (qph.dim + 1) * 8 + 4096 * 8
@@ -150,7 +159,7 @@ def test_ties_go_to_the_lowest_index(engine):
 
 
 def test_oversized_query_is_rejected(setup):
-    eng, cent, qph, lab, dist = setup
+    eng, cent, qph, lab, dist = _resident(setup)
     say("oversized")
     from d2_kmeans_amd._lib import D2MIError
     srv = Server(eng, 0, max_m=100)
@@ -167,7 +176,7 @@ def test_unnormalised_weights_are_renormalised(setup):
     lower bounds assume unit mass; without this a bound could exceed the
     LP value and drop the true nearest centroid.)  A weight <= 1e-9 is
     refused."""
-    eng, cent, qph, lab, dist = setup
+    eng, cent, qph, lab, dist = _resident(setup)
     say("unnormalised weights")
     from d2_kmeans_amd._lib import D2MIError
     rng = np.random.default_rng(8)
