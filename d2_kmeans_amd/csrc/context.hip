@@ -81,6 +81,7 @@ extern "C" int d2mi_create(d2mi_ctx **out, int device) {
     return D2MI_EINVAL;
   }
   D2MI_HIP(hipSetDevice(device));
+  D2MI_TRY(d2mi_noalloc_setup(device));
   d2mi_ctx *c = new d2mi_ctx();
   c->device = device;
   hipDeviceProp_t prop;

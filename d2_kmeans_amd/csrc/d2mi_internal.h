@@ -121,4 +121,7 @@ struct d2mi_ctx {
 };
 
 int d2mi_check_ctx(d2mi_ctx *ctx);
+// takes the device address of the no-allocation scratch array of `device`
+// (which must be current); once per device, from d2mi_create (transport.hip)
+int d2mi_noalloc_setup(int device);
 PhaseDev *d2mi_phase(d2mi_ctx *ctx, int phase);

@@ -4,6 +4,7 @@
 // histogram ground-cost matrix, or from a precomputed FP64 cost array.
 #include <algorithm>
 #include <cstdlib>
+#include <mutex>
 
 #include "d2mi_internal.h"
 #include "lp_solver.h"
@@ -107,8 +108,48 @@ __global__ void k_zero_int(int *p, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0;
 }
+__global__ void k_iota2(int *a, int *b, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = b[i] = i;
+}
 
 }  // namespace
+
+// The device memory of d2mi_transport_batch_noalloc: one fixed array per GPU,
+// held until the process ends.  Host code never uses this name as an
+// address (it is a host-side stand-in there): the device address comes from
+// hipGetSymbolAddress, once per device, in d2mi_noalloc_setup.
+static __device__ __align__(256) unsigned char scratch[D2MI_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char *g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+int d2mi_noalloc_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) {
+    d2mi_set_error("device %d beyond the %d the no-allocation scratch table holds", device,
+                   kMaxDevices);
+    return D2MI_EINVAL;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return D2MI_OK;
+  int cur = -1;
+  D2MI_HIP(hipGetDevice(&cur));
+  if (cur != device) {
+    d2mi_set_error("d2mi_noalloc_setup: device %d is not current (%d is)", device, cur);
+    return D2MI_EINVAL;
+  }
+  void *p = nullptr;
+  D2MI_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  if (!p) {
+    d2mi_set_error("hipGetSymbolAddress returned no address for the scratch array");
+    return D2MI_EHIP;
+  }
+  g_scratch_ptr[device] = (unsigned char *)p;
+  return D2MI_OK;
+}
 
 static int env_int(const char *name, int dflt) {
   const char *v = getenv(name);
@@ -244,6 +285,74 @@ extern "C" int d2mi_transport_batch(d2mi_ctx *ctx, int n, int m, int nbatch, con
     D2MI_HIP(hipMemcpyAsync(pivots, dpiv.p, sizeof(int) * nbatch, hipMemcpyDeviceToHost, S));
   int st = 0;
   D2MI_HIP(hipMemcpyAsync(&st, dst.p, sizeof(int), hipMemcpyDeviceToHost, S));
+  D2MI_HIP(hipStreamSynchronize(S));
+  if (st) {
+    d2mi_set_error("a transport LP hit the pivot limit");
+    return D2MI_ENUMERIC;
+  }
+  return D2MI_OK;
+}
+
+extern "C" int d2mi_transport_batch_noalloc(d2mi_ctx *ctx, int n, int m, int nbatch,
+                                            const double *C, const double *wX, const double *wY,
+                                            double *out, int *pivots) {
+  D2MI_TRY(d2mi_check_ctx(ctx));
+  if (n <= 0 || m <= 0 || nbatch <= 0 || !C || !wX || !wY || !out) {
+    d2mi_set_error("d2mi_transport_batch_noalloc: bad arguments");
+    return D2MI_EINVAL;
+  }
+  // staging inside the scratch array, each region 256-byte aligned
+  const size_t nm = (size_t)n * m, nb = (size_t)nbatch;
+  auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+  const size_t oC = 0, oA = oC + up(sizeof(double) * nm * nb), oB = oA + up(sizeof(double) * n * nb),
+               oO = oB + up(sizeof(double) * m * nb), oP = oO + up(sizeof(double) * nb),
+               oJa = oP + up(sizeof(int) * nb), oJb = oJa + up(sizeof(int) * nb),
+               oS = oJb + up(sizeof(int) * nb), total = oS + 256;
+  if (total > (size_t)D2MI_NOALLOC_SCRATCH_BYTES) {
+    d2mi_set_error("d2mi_transport_batch_noalloc: %d LPs of %d x %d need %zu bytes of staging, "
+                   "above the fixed %u-byte scratch; split the batch or use "
+                   "d2mi_transport_batch", nbatch, n, m, total, D2MI_NOALLOC_SCRATCH_BYTES);
+    return D2MI_EINVAL;
+  }
+  if (!g_scratch_ptr[ctx->device]) D2MI_TRY(d2mi_noalloc_setup(ctx->device));
+  unsigned char *base = g_scratch_ptr[ctx->device];
+  double *dC = (double *)(base + oC), *dA = (double *)(base + oA), *dB = (double *)(base + oB),
+         *dout = (double *)(base + oO);
+  int *dpiv = (int *)(base + oP), *dja = (int *)(base + oJa), *djb = (int *)(base + oJb),
+      *dst = (int *)(base + oS);
+  hipStream_t S = ctx->stream;
+  D2MI_HIP(hipMemcpyAsync(dC, C, sizeof(double) * nm * nb, hipMemcpyHostToDevice, S));
+  D2MI_HIP(hipMemcpyAsync(dA, wX, sizeof(double) * n * nb, hipMemcpyHostToDevice, S));
+  D2MI_HIP(hipMemcpyAsync(dB, wY, sizeof(double) * m * nb, hipMemcpyHostToDevice, S));
+  // the array keeps the last call's contents: values, pivot counts and the
+  // status flag are accumulated into, so they start from zero (fill kernels)
+  k_iota2<<<(nbatch + 255) / 256, 256, 0, S>>>(dja, djb, nbatch);
+  D2MI_HIP(hipGetLastError());
+  D2MI_TRY(d2mi_zero_doubles(ctx, dout, nbatch));
+  D2MI_TRY(d2mi_zero_ints(ctx, dpiv, nbatch));
+  D2MI_TRY(d2mi_zero_ints(ctx, dst, 1));
+  LpArgs a{};
+  a.njobs = nbatch;
+  a.job_a = dja;
+  a.job_b = djb;
+  a.b_is_table = 1;
+  a.cost_mode = LP_COST_GLOBAL;
+  a.sA = n;
+  a.A_w = dA;
+  a.sB = m;
+  a.B_w = dB;
+  a.gcost = dC;
+  a.gstride = (long long)nm;
+  a.n_max = n;
+  a.m_max = m;
+  a.out = dout;
+  a.pivots = dpiv;
+  a.status = dst;
+  D2MI_TRY(d2mi_launch_transport(ctx, a));
+  D2MI_HIP(hipMemcpyAsync(out, dout, sizeof(double) * nbatch, hipMemcpyDeviceToHost, S));
+  if (pivots) D2MI_HIP(hipMemcpyAsync(pivots, dpiv, sizeof(int) * nbatch, hipMemcpyDeviceToHost, S));
+  int st = 0;
+  D2MI_HIP(hipMemcpyAsync(&st, dst, sizeof(int), hipMemcpyDeviceToHost, S));
   D2MI_HIP(hipStreamSynchronize(S));
   if (st) {
     d2mi_set_error("a transport LP hit the pivot limit");

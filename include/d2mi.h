@@ -147,6 +147,23 @@ int d2mi_pdist2(d2mi_ctx *ctx, int d, size_t n, size_t m, const double *A, const
 int d2mi_transport_batch(d2mi_ctx *ctx, int n, int m, int nbatch, const double *C,
                          const double *wX, const double *wY, double *out, int *pivots);
 
+/* Size of the fixed per-GPU device array behind
+ * d2mi_transport_batch_noalloc (32 MiB). */
+#define D2MI_NOALLOC_SCRATCH_BYTES (32u << 20)
+
+/* d2mi_transport_batch without any device allocation.  No counterpart in
+ * the reference: it is for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments and results as d2mi_transport_batch; the device staging of
+ * the costs, weights, values, pivot counts, job lists and status lives in
+ * one fixed array of D2MI_NOALLOC_SCRATCH_BYTES per GPU, held until the
+ * process ends.  A batch needing more (about 8*nbatch*(n*m + n + m + 1) +
+ * 12*nbatch bytes plus alignment) is refused with D2MI_EINVAL.  Every call
+ * on a GPU shares that GPU's array, so calls that overlap on one GPU (from
+ * two threads or two contexts) race and must be serialised by the caller. */
+int d2mi_transport_batch_noalloc(d2mi_ctx *ctx, int n, int m, int nbatch, const double *C,
+                                 const double *wX, const double *wY, double *out, int *pivots);
+
 /* pdist2 on the matrix cores (_dpdist2, blas_like64.c:199-208, as a GEMM):
  * C[i*n + j] = |A_j - B_i|^2 for A: n x d, B: m x d (row-major, host), from
  * FP32 inputs with FP32 norms and accumulation; bf16 = 1 runs the products
