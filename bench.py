@@ -15,6 +15,11 @@ inner pass over one object's s x m_i plan).  It counts only when the FP32
 objective stays within 1e-5 of an FP64 run from the same state
 (``gate_obj_rel_err``).
 
+The ``clustering_loop`` object runs the whole clustering loop (Elkan
+labeling with exact LPs, B-ADMM, label switching) for a few outer
+iterations at the same N in FP64 and FP32, with the final labels of a
+random sample checked against the FP64 oracle's brute force.
+
 The ``assign`` object is the serving half of the metric: exact
 nearest-centroid queries (the reference's --eval assignment) against K=64
 resident centroids of 48 points in d=64; query sizes lognormal(ln 50, 1)
@@ -61,21 +66,6 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-class _Quiet:
-    """Silences C-level stdout (the reference build prints its tables)."""
-
-    def __enter__(self):
-        sys.stdout.flush()
-        self.fd = os.dup(1)
-        self.null = os.open(os.devnull, os.O_WRONLY)
-        os.dup2(self.null, 1)
-
-    def __exit__(self, *a):
-        os.dup2(self.fd, 1)
-        os.close(self.fd)
-        os.close(self.null)
-
-
 def profiled_traffic():
     """HBM bytes per plan-kernel launch measured by rocprofv3 PMC counters
     (FETCH_SIZE, WRITE_SIZE, scaled by a copy kernel) and committed under
@@ -90,29 +80,94 @@ def profiled_traffic():
     return None
 
 
-def cpu_baseline(data, labels, c0, opts, n_sample):
-    """The reference's own B-ADMM (oracle/_ref, built from its sources,
-    serial FP64, MKL sequential) on the first n_sample objects of the same
-    workload; falls back to this project's FP64 restatement."""
-    from oracle import oracle as O
-    sub = data.subset(np.arange(n_sample))
-    ph = sub.phases[0]
-    lab = labels[:n_sample].copy()
-    lsw = np.ones(n_sample, np.int8)
-    kind = "reference" if O.ref_available() else "port"
-    c = c0.copy()
-    Z = np.zeros(c.s * ph.col)
+def _cpu_info():
+    """(CPU model, nproc, cores this process may run on, cores used): the
+    cores used are the affinity set, capped by OMP_NUM_THREADS where the
+    environment sets one (a GPU box's host share)."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    aff = len(os.sched_getaffinity(0))
+    share = os.environ.get("OMP_NUM_THREADS")
+    use = min(aff, int(share)) if share and share.isdigit() and int(share) > 0 else aff
+    return model, os.cpu_count(), aff, max(1, use)
+
+
+def _cpu_shards(ph, labels, c0, opts, idx_list, tmp):
+    """Runs one oracle.cpu_shard process per shard, side by side, and
+    returns (wall seconds from the common start to the last finish, per-shard
+    reports).  The shards load their data first; the clock starts when all
+    are ready."""
+    import subprocess
+    procs = []
+    for r, idx in enumerate(idx_list):
+        sub = ph.subset(idx)
+        fn = os.path.join(tmp, f"shard{r}.npz")
+        np.savez(fn, dim=sub.dim, p_str=sub.p_str, p_w=sub.p_w, p_supp=sub.p_supp,
+                 label=np.asarray(labels)[idx].astype(np.int32), K=c0.K, s=c0.s, c_w=c0.p_w,
+                 c_supp=c0.p_supp, maxIters=opts.maxIters, rhoCoeff=opts.rhoCoeff,
+                 updatePerLoops=opts.updatePerLoops)
+        env = dict(os.environ, OMP_NUM_THREADS="1", MKL_NUM_THREADS="1")
+        procs.append(subprocess.Popen([sys.executable, "-m", "oracle.cpu_shard", fn], cwd=ROOT,
+                                      stdin=subprocess.PIPE, stdout=subprocess.PIPE, text=True,
+                                      env=env))
+    for p in procs:
+        line = p.stdout.readline()
+        if line.strip() != "ready":
+            raise RuntimeError(f"cpu shard did not start: {line!r}")
     t0 = time.perf_counter()
-    if kind == "reference":
-        with _Quiet():
-            O.ref_badmm(ph, lab, lsw, c, Z, opts)
-    else:
-        O.badmm(ph, lab, lsw, c, Z, opts)
-    dt = time.perf_counter() - t0
-    return {"value": n_sample * (opts.maxIters + 1) / dt, "unit": "plan updates/s", "cores": 1,
-            "kind": kind, "seconds": dt,
-            "sample": f"first {n_sample} objects of the same workload (s=40, K=10), one "
-                      f"{opts.maxIters + 1}-pass B-ADMM update, serial FP64"}
+    for p in procs:
+        p.stdin.write("go\n")
+        p.stdin.flush()
+    reps = []
+    for p in procs:
+        out, _ = p.communicate(timeout=600)
+        if p.returncode != 0:
+            raise RuntimeError(f"cpu shard failed with {p.returncode}")
+        res = [l for l in out.splitlines() if l.startswith("RESULT ")]
+        if not res:
+            raise RuntimeError(f"cpu shard printed no result: {out[-500:]!r}")
+        reps.append(json.loads(res[-1][len("RESULT "):]))
+    return time.perf_counter() - t0, reps
+
+
+def cpu_baseline(data, labels, c0, opts, n_one, n_all):
+    """The reference's own B-ADMM (oracle/_ref, built from its sources,
+    serial FP64, MKL sequential) timed on this host: on one core over the
+    first n_one objects, and on every core of the host share at once, one
+    process per core over disjoint shards of the first n_all objects (the
+    reference's MPI split without its per-pass allreduce).  Falls back to
+    this project's FP64 restatement where the reference build is absent."""
+    import shutil
+    import tempfile
+    ph = data.phases[0]
+    model, nproc, aff, P = _cpu_info()
+    passes = opts.maxIters + 1
+    tmp = tempfile.mkdtemp(prefix="d2mi_cpu_")
+    try:
+        n_one = min(n_one, ph.size)
+        w1, r1 = _cpu_shards(ph, labels, c0, opts, [np.arange(n_one)], tmp)
+        n_all = min(n_all, ph.size)
+        bounds = np.linspace(0, n_all, P + 1).astype(np.int64)
+        wP, rP = _cpu_shards(ph, labels, c0, opts,
+                             [np.arange(bounds[r], bounds[r + 1]) for r in range(P)], tmp)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    kind = r1[0]["kind"]
+    return {"value": n_all * passes / wP, "unit": "plan updates/s", "cores": P, "kind": kind,
+            "seconds": wP, "cpu_model": model, "nproc": nproc, "affinity_cores": aff,
+            "sample": f"all {P} cores: the first {n_all} objects of the same workload (s=40, "
+                      f"K=10) in {P} disjoint shards, one serial FP64 process each running one "
+                      f"{passes}-pass B-ADMM update (the MPI split without its allreduce); "
+                      f"the clock runs from the common start to the last shard's end",
+            "shard_seconds_max": max(r["seconds"] for r in rP),
+            "one_core": {"value": n_one * passes / w1, "cores": 1, "seconds": w1,
+                         "sample": f"first {n_one} objects, one {passes}-pass update"}}
 
 
 def oracle_assign(qph, cent, idx):
@@ -345,6 +400,71 @@ def multiphase_leg(eng, args):
     return out
 
 
+def cluster_leg(eng, args, data=None, c0=None):
+    """The whole clustering loop (d2_clustering, clustering.c:409-525) on the
+    image-like workload at full size: Elkan labeling with exact transport
+    LPs, B-ADMM, label switching and bound relaxation, a few outer
+    iterations from one starting centroid per stroke class, in FP64 and in
+    FP32 on the device.  Reported: time per stage, label changes and exact
+    distances per outer iteration, whether both precisions give the same
+    labels, and the final labels of a random sample checked against the
+    FP64 oracle's brute force against the final centroids."""
+    from d2_kmeans_amd import synth
+    from d2_kmeans_amd.data import BADMMOptions
+    from oracle import oracle as O
+    K, s = 10, 40
+    opts = BADMMOptions(10, 2.0, 10)
+    if data is None:
+        data, meta = synth.grid_strokes(n=args.n)
+        c0 = synth.seed_centroids(data.phases[0], K, s, labels=meta["classes"])
+    N = data.size
+    eng.set_data(data)
+    runs = {}
+    for precision in (64, 32):
+        eng.set_centroids(0, c0)
+        eng.set_labels(np.full(N, -1, np.int32), np.zeros(N, np.int8))
+        eng.synchronize()
+        t0 = time.perf_counter()
+        st = eng.clustering(K, args.cluster_iters, opts, use_triangle=True, precision=precision)
+        eng.synchronize()
+        dt = time.perf_counter() - t0
+        lab = eng.get_labels()[0]
+        runs[precision] = (st, dt, lab, eng.get_centroids(0))
+        log(f"[cluster] fp{precision}: {dt:.2f} s, changes {st['changes']}, "
+            f"distances {st['dist_count']}, labeling {st['ms_labeling']:.0f} ms, "
+            f"B-ADMM {st['ms_badmm']:.0f} ms")
+    st64, dt64, lab64, c64 = runs[64]
+    st32, dt32, lab32, c32 = runs[32]
+    rng = np.random.default_rng(synth.SEED_BASE + 50)
+    idx = np.sort(rng.choice(N, min(N, args.cluster_check), replace=False))
+    sub = data.subset(idx)
+    log(f"[cluster] FP64 oracle brute force on {len(idx)} sampled objects x {K} centroids")
+    lo64, _, _, _ = O.labeling(sub, [c64], label=np.full(len(idx), -1, np.int32))
+    lo32, _, _, _ = O.labeling(sub, [c32], label=np.full(len(idx), -1, np.int32))
+
+    def run_out(st, dt):
+        its = st["iterations"]
+        return {"seconds": dt, "iterations": its, "changes": st["changes"],
+                "dist_count": st["dist_count"],
+                "lps_per_object": [d / N for d in st["dist_count"]],
+                "ms_labeling": st["ms_labeling"], "ms_badmm": st["ms_badmm"],
+                "final_mean_cost": st["final_cost"]}
+
+    out = {"workload": f"grid-strokes D2 (N={N}, K={K}, s={s}), {args.cluster_iters} outer "
+                       "iterations from one object per stroke class, Elkan labeling, B-ADMM "
+                       "maxIters=10",
+           "fp64": run_out(st64, dt64), "fp32": run_out(st32, dt32),
+           "labels_fp32_equal_fp64": bool((lab32 == lab64).all()),
+           "label_agreement_fp32_fp64": float(np.mean(lab32 == lab64)),
+           "final_cost_rel_err_fp32": abs(st32["final_cost"] - st64["final_cost"]) /
+                                      abs(st64["final_cost"]),
+           "oracle_check": {"objects": int(len(idx)),
+                            "fp64_labels_equal": bool((lo64 == lab64[idx]).all()),
+                            "fp32_labels_equal": bool((lo32 == lab32[idx]).all())}}
+    log(f"[cluster] {out}")
+    return out
+
+
 def headline_leg(eng, args):
     """B-ADMM plan updates/s on the image-like workload, gated against FP64."""
     from d2_kmeans_amd import synth
@@ -449,11 +569,13 @@ def headline_leg(eng, args):
 
     cpu = None
     if not args.no_cpu_baseline:
-        log(f"[bench] cpu baseline on {args.cpu_sample} objects")
-        cpu = cpu_baseline(data, labels, c0, opts, args.cpu_sample)
+        log(f"[bench] cpu baseline: {args.cpu_sample} objects on 1 core, "
+            f"{args.cpu_sample_all} on every core")
+        cpu = cpu_baseline(data, labels, c0, opts, args.cpu_sample, args.cpu_sample_all)
         log(f"[bench] cpu baseline: {cpu}")
 
     return dict(value=value, ms_per_step=ms_per_step, ph=ph, K=K, s=s, elems=elems,
+                data=data, c0=c0,
                 achieved=achieved, avg_plan_ms=avg_plan_ms, bytes_per_launch=bytes_per_launch,
                 gate=gate, plain=plain, cpu=cpu, plan_forms=st["plan_forms"], graph=graph_used,
                 tune_report=tune_report,
@@ -466,7 +588,10 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=200_000)
-    ap.add_argument("--cpu-sample", type=int, default=6000)
+    ap.add_argument("--cpu-sample", type=int, default=12000,
+                    help="objects of the one-core CPU baseline")
+    ap.add_argument("--cpu-sample-all", type=int, default=200_000,
+                    help="objects of the all-cores CPU baseline (split over the cores)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-gate", action="store_true")
@@ -476,6 +601,9 @@ def main():
     ap.add_argument("--no-assign", action="store_true")
     ap.add_argument("--no-word-embed", action="store_true")
     ap.add_argument("--no-multiphase", action="store_true")
+    ap.add_argument("--no-cluster", action="store_true")
+    ap.add_argument("--cluster-iters", type=int, default=5)
+    ap.add_argument("--cluster-check", type=int, default=200)
     ap.add_argument("--multiphase-n", type=int, default=100_000)
     ap.add_argument("--assign-n", type=int, default=3000)
     ap.add_argument("--assign-qps", type=float, default=1000.0)
@@ -507,6 +635,7 @@ def main():
 
     eng = Engine(0)
     h = None if args.skip_badmm else headline_leg(eng, args)
+    clus = None if args.no_cluster else cluster_leg(eng, args, *((h["data"], h["c0"]) if h else ()))
     assign = None if args.no_assign else assign_leg(eng, args)
     word = None if args.no_word_embed else word_embed_leg(eng)
     multi = None if args.no_multiphase else multiphase_leg(eng, args)
@@ -541,6 +670,7 @@ def main():
         "plain_form": hd.get("plain"),
         "speedup_vs_plain": None if not hd.get("plain") else hd["value"] / hd["plain"]["value"],
         "cpu_baseline": hd.get("cpu"),
+        "clustering_loop": clus,
         "assign": assign,
         "word_embed": word,
         "multiphase": multi,

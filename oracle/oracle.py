@@ -463,12 +463,21 @@ def ref_prefill_hist_cost(phase, s, Cb):
                               Cb[s * off:].ctypes.data_as(_dp), int(phase.vocab_size), ptr(dm))
 
 
-def ref_badmm(phase, label, label_switch, cent, Z, opts):
+def ref_work(phase, K, s):
+    """The work arrays d2_allocate_work gives one phase (clustering_util.c:
+    152-230, centroid_Bregman.c:26-35): C, X, Y, Xc, Zr, all touched."""
+    N = phase.size
+    return dict(Cb=np.zeros(s * (phase.col + K * s)), X=np.zeros(s * phase.col),
+                Y=np.zeros(s * phase.col), Xc=np.zeros(phase.col), Zr=np.zeros(s * max(N, K)))
+
+
+def ref_badmm(phase, label, label_switch, cent, Z, opts, work=None):
     """Runs the reference's d2_centroid_sphBregman (centroid_Bregman.c:51)
     on one phase, in place on ``cent`` and ``Z``; returns (X, Y, C).
     Histogram phases get their cost blocks pre-filled first, as the
     reference's work allocation does (``ref_prefill_hist_cost``); a fresh
-    block per call, so every call starts from the raw ground cost."""
+    block per call, so every call starts from the raw ground cost.  ``work``
+    (from ``ref_work``) supplies the work arrays instead of fresh ones."""
     L = ref()
     ref_set_badmm_options(opts)
     keep = _Keep()
@@ -479,12 +488,11 @@ def ref_badmm(phase, label, label_switch, cent, Z, opts):
     m = RefMph(1, N, N, ptr(lab, _ip), cent.K, sph_arr)
     s = cent.s
     sc = s * phase.col
-    Cb = np.zeros(s * (phase.col + cent.K * s))
-    ref_prefill_hist_cost(phase, s, Cb)
-    X = np.zeros(sc)
-    Y = np.zeros(sc)
-    Xc = np.zeros(phase.col)
-    Zr = np.zeros(s * max(N, cent.K))
+    w = work if work is not None else ref_work(phase, cent.K, s)
+    Cb, X, Y, Xc, Zr = w["Cb"], w["X"], w["Y"], w["Xc"], w["Zr"]
+    if phase.metric in (5, 12):
+        Cb[:] = 0.0
+        ref_prefill_hist_cost(phase, s, Cb)
     gv = (RefVarSph * 1)(RefVarSph(ptr(Cb), None, None))
     bv = (RefVarBregman * 1)(RefVarBregman(ptr(X), ptr(Z), ptr(Y), ptr(Xc), ptr(Zr)))
     lsw = np.ascontiguousarray(label_switch, np.int8).copy()

@@ -149,3 +149,47 @@ def test_large_objects_d64_match_oracle(engine):
     assert rel(c64.p_w, co.p_w) < 1e-9
     assert rel(c64.p_supp, co.p_supp) < 1e-9
     assert abs(st64["obj"] - ro["obj"]) <= 1e-9 * abs(ro["obj"])
+
+
+def test_word_embed_update_at_configured_shape(engine):
+    """Word-embedding B-ADMM at the configured vocabulary and centroid
+    shape: V = 50,000 words of d = 300 (N(0,1) rows, L2-normalised), Zipf
+    document lengths in [10, 200], s = 64 support points per centroid,
+    K = 6, on 300 documents (centroid_Bregman.c:229-263 with _dpdist2_sym,
+    blas_like64.c:210-220).  FP64 agrees with the oracle to 1e-9; the FP32
+    update (cost table from the MFMA pdist2 with FP32 products, several
+    k-slices of the 300-long rows) and the same with BF16 products keep the
+    objective within 1e-5 of the oracle's (the metric's gate); the centroid
+    supports stay within 1e-4 with FP32 products and within 1e-3 with BF16
+    products (8-bit significands: each product carries up to 2^-8 relative
+    error before the FP32 accumulation)."""
+    say("word embeddings at V=50k, d=300, s=64")
+    data, _ = synth.word_embed_docs(n=300, V=50_000, d=300, seed=29, s=64)
+    ph = data.phases[0]
+    K = 6
+    c0 = synth.seed_centroids(ph, K, 64)
+    rng = np.random.default_rng(3)
+    lab = (np.arange(ph.size) % K).astype(np.int32)
+    rng.shuffle(lab)
+    lsw = np.ones(ph.size, np.int8)
+    opts = BADMMOptions(10, 2.0, 10)
+    co = c0.copy()
+    ro = O.badmm(ph, lab, lsw, co, np.zeros(64 * ph.col), opts)
+    st64, c64 = device_update(engine, data, lab, lsw, c0, opts, 64)
+    say(f"fp64 obj {st64['obj']:.12g} oracle {ro['obj']:.12g}")
+    assert rel(c64.p_w, co.p_w) < 1e-9
+    assert rel(c64.p_supp, co.p_supp) < 1e-9
+    assert abs(st64["obj"] - ro["obj"]) <= 1e-9 * abs(ro["obj"])
+    for bf16 in (False, True):
+        if bf16:
+            os.environ["D2MI_PDIST2_BF16"] = "1"
+        try:
+            st32, c32 = device_update(engine, data, lab, lsw, c0, opts, 32)
+        finally:
+            os.environ.pop("D2MI_PDIST2_BF16", None)
+        err = abs(st32["obj"] - ro["obj"]) / abs(ro["obj"])
+        say(f"fp32 {'bf16' if bf16 else 'fp32'} products: obj rel err {err:.2e}, "
+            f"supp rel err {rel(c32.p_supp, co.p_supp):.2e}")
+        assert err <= 1e-5
+        assert rel(c32.p_supp, co.p_supp) < (1e-3 if bf16 else 1e-4)
+        assert rel(c32.p_w, co.p_w) < (1e-3 if bf16 else 1e-4)
