@@ -29,9 +29,6 @@
 
 #include "d2mi_internal.h"
 
-int d2mi_pdist2_mfma_dev(hipStream_t S, int P, int Q, int d, const float *x, const float *y,
-                         float *nx, float *ny, float *T, int bf16);
-
 namespace {
 
 constexpr int NT = 256;        // threads per workgroup (4 waves of 64)
@@ -2501,16 +2498,36 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
   // (possibly captured) inner loop
   if (CM == COST_TABLE) D2MI_TRY(ctx->rec2.ensure(sizeof(double) * (size_t)(tab_G + K) * s * dsup));
   const bool mfma_words = word && !F64 && !getenv_flag("D2MI_WORD_DIRECT");
+  const bool bf16_words = mfma_words && getenv_flag("D2MI_PDIST2_BF16");
   if (CM == COST_TABLE && mfma_words) {
     D2MI_TRY(ctx->wtab.ensure(sizeof(float) * (size_t)p->V * K * s));
     D2MI_TRY(ctx->wnorm.ensure(sizeof(float) * ((size_t)p->V + (size_t)K * s)));
+    if (bf16_words) {
+      // the vocabulary in BF16 once per upload; centroid supports per rebuild
+      const size_t dp = d2mi_pdist2_pad(dsup);
+      D2MI_TRY(ctx->cs16.ensure(2 * (size_t)K * s * dp));
+      if (!p->vocab16.p) {
+        D2MI_TRY(p->vocab16.ensure(2 * (size_t)p->V * dp));
+        D2MI_TRY(p->vnorm16.ensure(sizeof(float) * (size_t)p->V));
+        D2MI_TRY(d2mi_rows_to_bf16_dev(ctx->stream, p->V, dsup, (const float *)p->vocab32.p,
+                                       p->vocab16.as<__bf16>(), p->vnorm16.as<float>()));
+      }
+    }
   }
   auto build_table = [&](hipStream_t S) -> int {
     if (mfma_words) {  // one GEMM for every (word, centroid point) pair, then a gather
       float *nv = ctx->wnorm.as<float>(), *nc = nv + p->V;
-      D2MI_TRY(d2mi_pdist2_mfma_dev(S, p->V, K * s, dsup, (const float *)p->vocab32.p,
-                                    (const float *)csT, nv, nc, ctx->wtab.as<float>(),
-                                    getenv_flag("D2MI_PDIST2_BF16") ? 1 : 0));
+      if (bf16_words) {
+        D2MI_TRY(d2mi_rows_to_bf16_dev(S, K * s, dsup, (const float *)csT,
+                                       ctx->cs16.as<__bf16>(), nc));
+        D2MI_TRY(d2mi_pdist2_bf16_dev(S, p->V, K * s, dsup, p->vocab16.as<__bf16>(),
+                                      p->vnorm16.as<float>(), ctx->cs16.as<__bf16>(), nc,
+                                      (const float *)p->vocab32.p, (const float *)csT,
+                                      ctx->wtab.as<float>()));
+      } else {
+        D2MI_TRY(d2mi_pdist2_mfma_dev(S, p->V, K * s, dsup, (const float *)p->vocab32.p,
+                                      (const float *)csT, nv, nc, ctx->wtab.as<float>()));
+      }
       k_word_gather<<<std::min(N, ctx->num_cu * 16), NT, 0, S>>>(
           N, s, K * s, p->p_str.as<int>(), p->cum.as<long long>(), ctx->label.as<int>(), sym,
           ctx->wtab.as<float>(), (float *)ctab);

@@ -96,7 +96,7 @@ extern "C" int d2mi_create(d2mi_ctx **out, int device) {
 
 static void release_phase(PhaseDev &p) {
   DevMem *all[] = {&p.p_str, &p.cum, &p.w64, &p.w32, &p.supp64, &p.supp32, &p.sym,
-                   &p.vocab64, &p.vocab32, &p.dist64, &p.dist32, &p.cw64, &p.cw32,
+                   &p.vocab64, &p.vocab32, &p.vocab16, &p.vnorm16, &p.dist64, &p.dist32, &p.cw64, &p.cw32,
                    &p.cs64, &p.cs32, &p.Z32, &p.Y32, &p.Z64, &p.Y64, &p.cw_old, &p.cs_old,
                    &p.Ctab, &p.Xout, &p.Z0, &p.bucket_map, &p.xmean, &p.xspread};
   for (DevMem *m : all) m->release();
@@ -114,7 +114,7 @@ extern "C" int d2mi_destroy(d2mi_ctx *ctx) {
                    &ctx->el_u, &ctx->el_s, &ctx->el_c, &ctx->el_r, &ctx->dist, &ctx->mask,
                    &ctx->jobs, &ctx->lpstat, &ctx->cnt, &ctx->mind2, &ctx->red,
                    &ctx->rec2, &ctx->rxbuf, &ctx->bkey, &ctx->permk, &ctx->blkcnt2,
-                   &ctx->segk, &ctx->wtab, &ctx->wnorm, &ctx->tuneZ, &ctx->tuneY,
+                   &ctx->segk, &ctx->wtab, &ctx->wnorm, &ctx->cs16, &ctx->tuneZ, &ctx->tuneY,
                    &ctx->rowsum};
   for (DevMem *m : all) m->release();
   for (auto &e : ctx->ev)

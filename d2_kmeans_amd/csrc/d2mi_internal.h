@@ -53,6 +53,8 @@ struct PhaseDev {
                                 // and sum of squared distances to it [N]
   DevMem sym;                   // int[col] word ids / histogram bins
   DevMem vocab64, vocab32;      // [V*dim]
+  DevMem vocab16, vnorm16;      // word embeddings: BF16 rows padded to d2mi_pdist2_pad(dim),
+                                // FP32 norms of the rounded rows (made on first use)
   DevMem dist64, dist32;        // [V*V]
   // centroids (K x s)
   int K = 0, s = 0;
@@ -97,6 +99,7 @@ struct d2mi_ctx {
   DevMem rec2, rxbuf;           // support-numerator records, X row sums per label
   DevMem bkey, permk, blkcnt2, segk;  // B-ADMM order by (object-size bucket, label)
   DevMem wtab, wnorm;           // word embeddings: pdist2(vocab, centroid supports), norms
+  DevMem cs16;                  // BF16 centroid supports for the BF16 pdist2
   DevMem rowsum;                // histogram phases: ground-cost line sums for rho
   // pdist2 scratch
   DevMem pA, pB, pC;
@@ -121,6 +124,20 @@ struct d2mi_ctx {
 };
 
 int d2mi_check_ctx(d2mi_ctx *ctx);
+
+// pdist2 tables on the matrix cores (pdist2.hip): T (P x Q, FP32) from FP32
+// rows x (P x d) and y (Q x d) with FP32 products (nx, ny: norm scratch), or
+// with BF16 products from rows padded to d2mi_pdist2_pad(d) and the FP32
+// norms (d2mi_rows_to_bf16_dev makes both; x and y stay the FP32 rows for
+// the cancellation recompute)
+int d2mi_pdist2_mfma_dev(hipStream_t S, int P, int Q, int d, const float *x, const float *y,
+                         float *nx, float *ny, float *T);
+int d2mi_pdist2_pad(int d);
+int d2mi_row_norms_dev(hipStream_t S, int R, int d, const float *x, float *nx);
+int d2mi_rows_to_bf16_dev(hipStream_t S, int R, int d, const float *x, __bf16 *xb, float *nx);
+int d2mi_pdist2_bf16_dev(hipStream_t S, int P, int Q, int d, const __bf16 *xb, const float *nx,
+                         const __bf16 *yb, const float *ny, const float *x, const float *y,
+                         float *T);
 // takes the device address of the no-allocation scratch array of `device`
 // (which must be current); once per device, from d2mi_create (transport.hip)
 int d2mi_noalloc_setup(int device);

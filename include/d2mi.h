@@ -167,8 +167,10 @@ int d2mi_transport_batch_noalloc(d2mi_ctx *ctx, int n, int m, int nbatch, const 
 /* pdist2 on the matrix cores (_dpdist2, blas_like64.c:199-208, as a GEMM):
  * C[i*n + j] = |A_j - B_i|^2 for A: n x d, B: m x d (row-major, host), from
  * FP32 inputs with FP32 norms and accumulation; bf16 = 1 runs the products
- * in BF16.  Entries small next to |A_j|^2 + |B_i|^2 are recomputed by direct
- * differences.  ms (may be NULL) = time of the GEMM kernel. */
+ * in BF16 (operands rounded to BF16 once, norms of the FP32 rows).  Entries
+ * small next to |A_j|^2 + |B_i|^2 are recomputed by direct differences.
+ * ms (may be NULL) = time of the GEMM kernel alone (norms and the BF16
+ * conversion run before it). */
 int d2mi_pdist2_mfma(d2mi_ctx *ctx, int d, size_t n, size_t m, const double *A, const double *B,
                      double *C, int bf16, float *ms);
 

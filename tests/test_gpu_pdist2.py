@@ -1,7 +1,8 @@
 """pdist2 on the matrix cores against the FP64 oracle (_dpdist2,
 blas_like64.c:199-208).  FP32 products: relative error within 1e-5 of the
 FP64 table (FP32 rounding of |x|^2 + |y|^2 - 2 x.y, with near-zero entries
-recomputed by direct differences); BF16 products: within 2e-2."""
+recomputed by direct differences); BF16 products: the exact squared distance
+of the BF16-rounded rows to FP32 accumulation error."""
 import numpy as np
 import pytest
 
@@ -32,13 +33,41 @@ def test_pdist2_mfma_fp32(engine, n, m, d):
     assert (got >= 0).all()
 
 
-def test_pdist2_mfma_bf16(engine):
-    say("pdist2 mfma bf16")
-    rng = np.random.default_rng(3)
-    A = rng.standard_normal((256, 300))
-    B = rng.standard_normal((512, 300))
+def bf16_round(x):
+    """FP64 -> FP32 -> BF16 with round-to-nearest-even, back as FP64."""
+    b = np.ascontiguousarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    b = (b + 0x7FFF + ((b >> 16) & 1)) & 0xFFFF0000
+    return b.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+@pytest.mark.parametrize("n,m,d", [(256, 512, 300), (77, 129, 13), (1, 5, 1), (200, 64, 64),
+                                   (130, 1300, 40)])
+def test_pdist2_mfma_bf16(engine, n, m, d):
+    """BF16 products: every entry is |x|^2 + |y|^2 - 2 bf16(x).bf16(y) with
+    the norms of the FP32 rows and the products accumulated in FP32, so
+    against that expression in FP64 it is exact to FP32 accumulation error,
+    1e-5 of |x|^2 + |y|^2; entries small next to that (near-duplicate rows)
+    are recomputed from the FP32 rows by direct differences and agree with
+    the FP64 oracle on the unrounded rows to 1e-5 relative."""
+    say(f"pdist2 mfma bf16 {n}x{m}x{d}")
+    rng = np.random.default_rng(3 + n + d)
+    A = rng.standard_normal((n, d))
+    B = rng.standard_normal((m, d))
     A /= np.linalg.norm(A, axis=1, keepdims=True)
     B /= np.linalg.norm(B, axis=1, keepdims=True)
-    ref = O.pdist2(A, B)
-    got = engine.pdist2_mfma(A, B, bf16=True)
-    assert np.max(np.abs(got - ref) / np.maximum(ref, 1e-2)) < 2e-2
+    k = min(n, m) // 3
+    B[:k] = A[:k] + 1e-4 * rng.standard_normal((k, d))
+    got = engine.pdist2_mfma(A, B, bf16=True).reshape(m, n)
+    A32 = A.astype(np.float32).astype(np.float64)
+    B32 = B.astype(np.float32).astype(np.float64)
+    A16, B16 = bf16_round(A), bf16_round(B)
+    norms = (A32 ** 2).sum(1)[None, :] + (B32 ** 2).sum(1)[:, None]
+    ref16 = norms - 2.0 * (B16 @ A16.T)  # [i, j]: |A_j|^2 + |B_i|^2 - 2 A16_j . B16_i
+    ref = O.pdist2(A32, B32).reshape(m, n)
+    far = ref16 > 0.02 * norms
+    near = ref16 < 0.005 * norms
+    assert far.any()
+    assert np.max(np.abs(got - ref16)[far] / norms[far]) < 1e-5
+    if near.any():
+        assert np.max(np.abs(got - ref)[near] / np.maximum(ref[near], 1e-7)) < 1e-5
+    assert (got >= 0).all()
