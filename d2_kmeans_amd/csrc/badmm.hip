@@ -98,13 +98,14 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 // Block-wide sum with a fixed association order; every thread gets the
 // result.  Must be called by all threads of the block.
+template <int NTB = NT>
 __device__ double block_sum(double v, double *red) {
   v = wave_sum(v);
   __syncthreads();
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
   __syncthreads();
   double t = 0.0;
-  for (int i = 0; i < NT / 64; ++i) t += red[i];
+  for (int i = 0; i < NTB / 64; ++i) t += red[i];
   return t;
 }
 
@@ -907,11 +908,12 @@ __device__ __forceinline__ void buf_store(T v, BufRes r, int voff, int soff) {
 // sums); column stride sp even, as for k_badmm_plan64.
 struct PlanRegLds {
   size_t A, col, w, xs, cs, cw, g, rp, ids, acc, red, up, total;
-  // noA: t lives in the prefetch form's static staging array instead
+  // noA: t lives in the prefetch form's static staging array instead;
+  // nt: threads per workgroup
   __host__ __device__ PlanRegLds(int tsize, int sp, int m_max, int s, int d, int R,
-                                 bool noA = false) {
+                                 bool noA = false, int nt = NT) {
     const size_t dd = d > 0 ? d : 1;
-    const int cpi = NT / s;
+    const int cpi = nt / s;
     size_t o = 0;
     A = o; o = align16(o + (noA ? 0 : (size_t)sp * m_max * tsize));
     col = o; o = align16(o + (size_t)m_max * tsize);
@@ -920,7 +922,7 @@ struct PlanRegLds {
     cs = o; o = align16(o + (size_t)s * dd * tsize);
     cw = o; o = align16(o + (size_t)s * tsize);
     g = o; o = align16(o + (size_t)s * tsize);
-    rp = o; o = align16(o + (size_t)NT * tsize);
+    rp = o; o = align16(o + (size_t)nt * tsize);
     ids = o; o = align16(o + (size_t)m_max * sizeof(int));
     acc = o; o = align16(o + (size_t)R * sizeof(double));
     red = o; o = align16(o + 8 * sizeof(double));
@@ -930,7 +932,9 @@ struct PlanRegLds {
 };
 
 // The plan update with each thread's plan elements held in registers from
-// load to store, for s <= 64 and objects of at most NC * (NT / s) supports.
+// load to store, for s <= 64 and objects of at most NC * (NTB / s) supports,
+// NTB threads per workgroup: 256, or 64 for small plans (the wave form: one
+// wave per object, so every barrier below is a single wave's).
 // Thread tid owns row k = tid % s of columns j = tid / s + c (NT / s),
 // c < NC, so the element offsets k + j s are contiguous over the workgroup.
 // Per object:
@@ -977,8 +981,8 @@ __device__ __forceinline__ void full_sync() {
   asm volatile("" ::: "memory");
 }
 
-template <typename T, int CM, int DIM, int NC, int MODE, bool PF = false>
-__global__ __launch_bounds__(NT)
+template <typename T, int CM, int DIM, int NC, int MODE, bool PF = false, int NTB = NT>
+__global__ __launch_bounds__(NTB)
 __attribute__((amdgpu_waves_per_eu(
     PF ? 4  // the staging arrays (36 KB) allow 4 workgroups per CU
     : MODE == 0 || (MODE == 1 && D2MI_PLANREG_MID_CAP && !std::is_same<T, double>::value)
@@ -988,6 +992,7 @@ void k_badmm_plan_reg(PlanArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr bool F64 = std::is_same<T, double>::value;
   static_assert(!PF || (MODE == 0 && !F64 && CM != COST_TABLE), "prefetch: lean FP32 passes");
+  static_assert(!PF || NTB == NT, "prefetch: 256-thread workgroups");
   __shared__ __align__(16) T stZ[PF ? PF_STAGE : 1], stY[PF ? PF_STAGE : 1];
   constexpr int U = PLAN_UNROLL;
   typedef typename Vec2<T>::type T2;
@@ -996,7 +1001,7 @@ void k_badmm_plan_reg(PlanArgs<T> a) {
   const bool upd = MODE >= 1 && a.upd, first = MODE >= 1 && a.first;
   const bool resid = MODE == 2 && a.resid;
   const bool keepx = MODE >= 1 && (F64 || resid);  // x held until pass 4
-  const PlanRegLds L(sizeof(T), sp, a.m_max, s, d, a.R, PF);
+  const PlanRegLds L(sizeof(T), sp, a.m_max, s, d, a.R, PF, NTB);
   T *A = PF ? stZ : (T *)(smem + L.A);
   T *csl = (T *)(smem + L.col);
   T *wl = (T *)(smem + L.w), *xs = (T *)(smem + L.xs);
@@ -1010,12 +1015,12 @@ void k_badmm_plan_reg(PlanArgs<T> a) {
   const int p0 = gi * a.chunk, p1 = min(a.N, p0 + a.chunk);
   const double rho = a.scal[0];
   const T inv_rho = (T)a.scal[1];
-  const int cpi = NT / s, ke_t = tid % s;
+  const int cpi = NTB / s, ke_t = tid % s;
   const int jb_t = tid < cpi * s ? tid / s : 0x3fffffff;  // idle threads own no column
   const int q4 = tid & 3, RQ = (((s + 3) >> 2) + 1) & ~1, r0 = q4 * RQ;
   const int nr = max(0, min(RQ, s - r0));  // pass 2: rows of this thread in a column
   double r_obj = 0.0, r_pr = 0.0, r_dr = 0.0;
-  for (int q = tid; q < a.R; q += NT) acc[q] = 0.0;
+  for (int q = tid; q < a.R; q += NTB) acc[q] = 0.0;
   int cur = -1;
   // PF: object pq's Z and Y blocks (s m contiguous floats each) into stZ, stY,
   // 64 elements per wave instruction (the LDS address is wave-uniform)
@@ -1025,7 +1030,7 @@ void k_badmm_plan_reg(PlanArgs<T> a) {
       const int mq = const_load(a.p_str + oq);
       const size_t bq = (size_t)s * const_load(a.cum + oq);
       const int n = s * mq;
-      for (int i = wv; i * 64 < n; i += NT / 64) {
+      for (int i = wv; i * 64 < n; i += NTB / 64) {
         const int e = i * 64 + lane;
         if (e < n) {
           __builtin_amdgcn_global_load_lds(a.Z + bq + e, stZ + i * 64, 4, 0, 0);
@@ -1082,7 +1087,7 @@ void k_badmm_plan_reg(PlanArgs<T> a) {
     int ipre[PRE_W];
 #pragma unroll
     for (int r = 0; r < PRE_W; ++r) {
-      const int q = tid + r * NT;
+      const int q = tid + r * NTB;
       if (q < m) {
         wpre[r] = a.w[off + q];
         if constexpr (CM == COST_SPARSE) ipre[r] = a.sym[off + q];
@@ -1091,25 +1096,25 @@ void k_badmm_plan_reg(PlanArgs<T> a) {
     if constexpr (CM == COST_EUCLID) {
 #pragma unroll
       for (int r = 0; r < PRE_X; ++r) {
-        const int q = tid + r * NT;
+        const int q = tid + r * NTB;
         if (q < m * d) xpre[r] = a.supp[(size_t)off * d + q];
       }
     }
     if (l != cur) {
       __syncthreads();  // every wave is done with the previous label
       if (cur >= 0)
-        for (int q = tid; q < a.R; q += NT) {
+        for (int q = tid; q < a.R; q += NTB) {
           a.rec[(size_t)(gi + cur) * a.R + q] = acc[q];
           acc[q] = 0.0;
         }
       cur = l;
-      for (int q = tid; q < s; q += NT) cwl[q] = a.cw[(size_t)l * s + q];
+      for (int q = tid; q < s; q += NTB) cwl[q] = a.cw[(size_t)l * s + q];
       if constexpr (CM == COST_EUCLID)
-        for (int q = tid; q < s * d; q += NT) cs[q] = a.csupp[(size_t)l * s * d + q];
+        for (int q = tid; q < s * d; q += NTB) cs[q] = a.csupp[(size_t)l * s * d + q];
     }
 #pragma unroll
     for (int r = 0; r < PRE_W; ++r) {
-      const int q = tid + r * NT;
+      const int q = tid + r * NTB;
       if (q < m) {
         wl[q] = wpre[r];
         if constexpr (CM == COST_SPARSE) ids[q] = ipre[r];
@@ -1118,7 +1123,7 @@ void k_badmm_plan_reg(PlanArgs<T> a) {
     if constexpr (CM == COST_EUCLID) {
 #pragma unroll
       for (int r = 0; r < PRE_X; ++r) {
-        const int q = tid + r * NT;
+        const int q = tid + r * NTB;
         if (q < m * d) xs[q] = xpre[r];
       }
     }
@@ -1169,7 +1174,7 @@ void k_badmm_plan_reg(PlanArgs<T> a) {
     }
     __syncthreads();
     // pass 2 -- step 1b: column sums (_dcnorm, :158) -> the column's scale
-    for (int j = tid >> 2; j < m; j += NT / 4) {
+    for (int j = tid >> 2; j < m; j += NTB / 4) {
       const T *colA = A + j * sp + r0;
       T part = 0;
       for (int r = 0; r < nr; r += 2) {
@@ -1289,11 +1294,11 @@ void k_badmm_plan_reg(PlanArgs<T> a) {
   }
   __syncthreads();
   if (cur >= 0)
-    for (int q = tid; q < a.R; q += NT) a.rec[(size_t)(gi + cur) * a.R + q] = acc[q];
+    for (int q = tid; q < a.R; q += NTB) a.rec[(size_t)(gi + cur) * a.R + q] = acc[q];
   if (resid) {
-    const double t0 = block_sum(r_obj, red);
-    const double t1 = block_sum(r_pr, red);
-    const double t2 = block_sum(r_dr, red);
+    const double t0 = block_sum<NTB>(r_obj, red);
+    const double t1 = block_sum<NTB>(r_pr, red);
+    const double t2 = block_sum<NTB>(r_dr, red);
     if (tid == 0) {
       a.resid_part[gi * 3 + 0] = t0;
       a.resid_part[gi * 3 + 1] = t1;
@@ -2304,7 +2309,8 @@ int occupancy_of(F kernel, size_t lds, int threads = NT) {
 // (plan in LDS, s <= 64), k_badmm_plan (any s) or its HBM-scratch form (GM);
 // LEAN instances serve passes after the first with no support update,
 // residual or kept X
-enum PlanKind { PK_LEGACY, PK_GM, PK_64, PK_REG8, PK_REG16, PK_TALL, PK_REG8P, PK_REG16P };
+enum PlanKind { PK_LEGACY, PK_GM, PK_64, PK_REG8, PK_REG16, PK_TALL, PK_REG8P, PK_REG16P,
+                PK_WAVE8, PK_WAVE16 };
 
 const char *plan_kind_name(PlanKind k) {
   switch (k) {
@@ -2315,6 +2321,8 @@ const char *plan_kind_name(PlanKind k) {
     case PK_TALL: return "tall";
     case PK_REG8P: return "reg8p";
     case PK_REG16P: return "reg16p";
+    case PK_WAVE8: return "wave8";
+    case PK_WAVE16: return "wave16";
     default: return "general";
   }
 }
@@ -2323,11 +2331,17 @@ inline bool is_tall(PlanKind k) { return k == PK_TALL; }
 // register forms; the P kinds run their lean passes with the next object's
 // plan copied into LDS ahead of time (k_badmm_plan_reg<..., PF = true>)
 inline bool is_reg(PlanKind k) {
-  return k == PK_REG8 || k == PK_REG16 || k == PK_REG8P || k == PK_REG16P;
+  return k == PK_REG8 || k == PK_REG16 || k == PK_REG8P || k == PK_REG16P || k == PK_WAVE8 ||
+         k == PK_WAVE16;
 }
 inline bool is_pf(PlanKind k) { return k == PK_REG8P || k == PK_REG16P; }
-// workgroup size of a plan kernel: one wave per 64 rows for the tall form
-inline int plan_threads(PlanKind k, int s) { return is_tall(k) ? (s + 63) / 64 * 64 : NT; }
+// the register form with 64-thread workgroups: one wave per object
+inline bool is_wave(PlanKind k) { return k == PK_WAVE8 || k == PK_WAVE16; }
+// workgroup size of a plan kernel: one wave per 64 rows for the tall form,
+// one wave for the wave forms
+inline int plan_threads(PlanKind k, int s) {
+  return is_tall(k) ? (s + 63) / 64 * 64 : is_wave(k) ? 64 : NT;
+}
 
 // d2mi_badmm_tune's request: passes timed per candidate, and the report
 struct TuneReq {
@@ -2390,6 +2404,17 @@ struct PlanKernels {
   static int occ(size_t lds, PlanKind k, int mode, int s, int mc) {
     switch (k) {
       case PK_TALL: return tall_occ(mc, LEAN ? 0 : mode, plan_threads(k, s));
+      case PK_WAVE16:
+        if constexpr (F32) {
+          if constexpr (LEAN) return occupancy_of(k_badmm_plan_reg<T, CM, DIM, 16, 0, false, 64>, lds, 64);
+          return mode == 2 ? occupancy_of(k_badmm_plan_reg<T, CM, DIM, 16, 2, false, 64>, lds, 64)
+                           : occupancy_of(k_badmm_plan_reg<T, CM, DIM, 16, 1, false, 64>, lds, 64);
+        }
+        return 0;
+      case PK_WAVE8:
+        if constexpr (LEAN) return occupancy_of(k_badmm_plan_reg<T, CM, DIM, 8, 0, false, 64>, lds, 64);
+        return mode == 2 ? occupancy_of(k_badmm_plan_reg<T, CM, DIM, 8, 2, false, 64>, lds, 64)
+                         : occupancy_of(k_badmm_plan_reg<T, CM, DIM, 8, 1, false, 64>, lds, 64);
       case PK_REG16P:
         if constexpr (PFOK && LEAN) return occupancy_of(k_badmm_plan_reg<T, CM, DIM, 16, 0, true>, lds);
         [[fallthrough]];
@@ -2417,6 +2442,18 @@ struct PlanKernels {
     const int mode = reg_mode<LEAN>(pa);
     switch (k) {
       case PK_TALL: tall_run(G, S, pa, mode); break;
+      case PK_WAVE16:
+        if constexpr (F32) {
+          if constexpr (LEAN) k_badmm_plan_reg<T, CM, DIM, 16, 0, false, 64><<<G, 64, lds, S>>>(pa);
+          else if (mode == 2) k_badmm_plan_reg<T, CM, DIM, 16, 2, false, 64><<<G, 64, lds, S>>>(pa);
+          else k_badmm_plan_reg<T, CM, DIM, 16, 1, false, 64><<<G, 64, lds, S>>>(pa);
+        }
+        break;
+      case PK_WAVE8:
+        if constexpr (LEAN) k_badmm_plan_reg<T, CM, DIM, 8, 0, false, 64><<<G, 64, lds, S>>>(pa);
+        else if (mode == 2) k_badmm_plan_reg<T, CM, DIM, 8, 2, false, 64><<<G, 64, lds, S>>>(pa);
+        else k_badmm_plan_reg<T, CM, DIM, 8, 1, false, 64><<<G, 64, lds, S>>>(pa);
+        break;
       case PK_REG16P:
         if constexpr (PFOK && LEAN) {
           k_badmm_plan_reg<T, CM, DIM, 16, 0, true><<<G, NT, lds, S>>>(pa);
@@ -2853,6 +2890,16 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
           return s <= 64 && ncols <= (k == PK_REG8 ? 8 : 16) &&
                  Plan64Lds(sizeof(T), sp64, m_lim, s, d, 1, Rmax).total <= ctx->lds_max &&
                  PlanRegLds(sizeof(T), sp64, m_lim, s, d, Rmax).total <= ctx->lds_max;
+        case PK_WAVE16:
+          if (F64) return false;
+          [[fallthrough]];
+        case PK_WAVE8: {  // one wave: 64 / s columns per slot, weights and supports 64 wide
+          if (s > 64) return false;
+          const int wcols = (m_lim + 64 / s - 1) / (64 / s);
+          return wcols <= (k == PK_WAVE8 ? 8 : 16) && m_lim <= 64 * PRE_W &&
+                 (size_t)m_lim * std::max(d, 1) <= (size_t)64 * PRE_X &&
+                 PlanRegLds(sizeof(T), sp64, m_lim, s, d, Rmax, false, 64).total <= ctx->lds_max;
+        }
         case PK_64:
           return s <= 64 && Plan64Lds(sizeof(T), sp64, m_lim, s, d, 1, Rmax).total <= ctx->lds_max;
         case PK_LEGACY:
@@ -2876,7 +2923,12 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
       } else if (!force_gm) {
         if (!no_64 && fits(B.m_max, PK_64)) {
           k = PK_64;
-          if (!no_reg && fits(B.m_max, PK_REG8)) k = PK_REG8;
+          // small plans (at most 8 elements per lane of one wave): the wave
+          // form, one object per wave; otherwise the 256-thread register form
+          const bool small = (size_t)s * B.m_max <= 512 && !getenv_flag("D2MI_PLAN_NO_WAVE");
+          if (!no_reg && small && fits(B.m_max, PK_WAVE8)) k = PK_WAVE8;
+          else if (!no_reg && small && fits(B.m_max, PK_WAVE16)) k = PK_WAVE16;
+          else if (!no_reg && fits(B.m_max, PK_REG8)) k = PK_REG8;
           else if (!no_reg && fits(B.m_max, PK_REG16)) k = PK_REG16;
           // D2MI_PLAN_PREFETCH=1: the register forms' prefetching variants
           if (getenv_flag("D2MI_PLAN_PREFETCH")) {
@@ -2902,7 +2954,8 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
     auto lds_of = [&](const Bucket &B, int keepx, int R, bool lean = false) -> size_t {
       if (is_tall(B.kind)) return 0;  // static LDS only
       if (is_reg(B.kind))
-        return PlanRegLds(sizeof(T), B.sp, B.m_max, s, d, R, lean && is_pf(B.kind)).total;
+        return PlanRegLds(sizeof(T), B.sp, B.m_max, s, d, R, lean && is_pf(B.kind),
+                          plan_threads(B.kind, s)).total;
       return B.k64 ? Plan64Lds(sizeof(T), B.sp, B.m_max, s, d, keepx, R).total
                    : PlanLds(sizeof(T), B.sp, B.m_max, s, d, keepx, R, B.gm).total;
     };
@@ -2976,8 +3029,8 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
         // than 3% faster on the plan passes, since more workgroups also cost
         // the per-pass reduction (more records) that these timings leave out
         std::vector<PlanTuneVal> cands{PlanTuneVal{(int)B.kind, 0}};
-        const PlanKind kinds[] = {PK_TALL, PK_REG8, PK_REG16, PK_REG8P, PK_REG16P, PK_64,
-                                  PK_LEGACY, PK_GM};
+        const PlanKind kinds[] = {PK_TALL, PK_WAVE8, PK_WAVE16, PK_REG8, PK_REG16, PK_REG8P,
+                                  PK_REG16P, PK_64, PK_LEGACY, PK_GM};
         bool any_fast = false;
         for (PlanKind k : kinds)
           if (k != PK_GM && fits(B.m_max, k)) any_fast = true;
