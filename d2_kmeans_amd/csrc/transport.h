@@ -47,6 +47,7 @@ struct LpArgs {
   int n_max, m_max;
   // launch shape, filled by d2mi_plan_transport
   int threads, cpt, table, stage_a, stage_b, vogel;
+  int dual_iters;  // entropic iterations for Vogel's reduced-cost start (0: plain start)
   size_t lds;
   int threads_hint;     // > 0: workgroup size to use
   hipStream_t stream;   // null: the context stream

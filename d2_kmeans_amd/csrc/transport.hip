@@ -86,6 +86,7 @@ __global__ __launch_bounds__(1024) void k_transport(LpArgs a) {
     P.tab_rel = 0x1p-22f;
     P.cpt = a.cpt;
     P.vogel = a.vogel;
+    P.dual_iters = a.dual_iters;
     int piv = 0;
     const double v = lp::solve(P, lds, L, &piv, a.status);
     if (tid == 0) {
@@ -205,6 +206,12 @@ int d2mi_plan_transport(d2mi_ctx *ctx, LpArgs &a) {
   // a Monge matrix, where the north-west corner is already optimal
   const bool hist = a.cost_mode == LP_COST_HIST || a.cost_mode == LP_COST_SPARSE;
   a.vogel = env_int("D2MI_LP_VOGEL", hist ? 0 : 1) != 0;
+  // Vogel on reduced costs from approximate duals, where the pivots it
+  // saves outweigh its entropic iterations (D2MI_LP_DUAL_MIN: cell threshold,
+  // D2MI_LP_DUAL_ITERS: iteration count, 0 = plain Vogel)
+  const long long dual_min = env_int("D2MI_LP_DUAL_MIN", lp::DUAL_MIN_CELLS);
+  const int dual_iters = env_int("D2MI_LP_DUAL_ITERS", lp::dual_iters_for((long long)cells));
+  a.dual_iters = a.vogel && tab && (long long)cells >= dual_min && dual_iters >= 2 ? dual_iters : 0;
   return D2MI_OK;
 }
 
