@@ -557,6 +557,11 @@ def headline_leg(eng, args):
     avg_plan_ms = plan_ms / plan_launches
     bytes_per_launch = 16.0 * elems  # Z and Y read + written once, FP32
     achieved = bytes_per_launch / (avg_plan_ms / 1000.0) / 1e9
+    # the practical ceiling for this access pattern on this GPU: two arrays of
+    # the plan's size streamed in place (read and written), no arithmetic
+    ceiling, ceiling_report = eng.stream_probe(elems, 5)
+    log(f"[bench] two-array in-place stream over {int(elems)} elements: best "
+        f"{ceiling:.1f} GB/s\n{ceiling_report}")
 
     # ---- plain one-thread-per-object form (the 20x denominator)
     plain = None
@@ -587,6 +592,7 @@ def headline_leg(eng, args):
     return dict(value=value, ms_per_step=ms_per_step, ph=ph, K=K, s=s, elems=elems,
                 data=data, c0=c0,
                 achieved=achieved, avg_plan_ms=avg_plan_ms, bytes_per_launch=bytes_per_launch,
+                ceiling=ceiling,
                 gate=gate, plain=plain, cpu=cpu, plan_forms=st["plan_forms"], graph=graph_used,
                 tune_report=tune_report,
                 eager={"value": ph.size * passes / (ms_eager / 1000.0), "ms_per_step": ms_eager})
@@ -671,7 +677,12 @@ def main():
             "kernel_timing": "HIP events around every plan launch of the same number of "
                              "updates launched one by one right after the timed region "
                              "(graph nodes cannot be timed with HIP events)",
-            "algorithmic_bytes_per_launch": hd["bytes_per_launch"]},
+            "algorithmic_bytes_per_launch": hd["bytes_per_launch"],
+            "stream_ceiling": {
+                "GBps": hd["ceiling"], "frac": hd["achieved"] / hd["ceiling"],
+                "what": "best of dword and float4 grid-stride kernels streaming two FP32 arrays "
+                        "of the plan's size in place (16 B per element), measured in the same run "
+                        "(d2mi_stream_probe); frac = achieved / this"}},
         "inner_loop_graph": hd.get("graph"),
         "eager_launches": hd.get("eager"),
         "tuner": hd.get("tune_report"),

@@ -77,6 +77,7 @@ SIGNATURES = {
     "d2mi_badmm_tune": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_char_p, C.c_size_t]),
     "d2mi_badmm_tune_clear": (C.c_int, [_vp]),
     "d2mi_get_plan": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _dp]),
+    "d2mi_stream_probe": (C.c_int, [_vp, C.c_size_t, C.c_int, _dp, C.c_char_p, C.c_int]),
     "d2mi_set_plan": (C.c_int, [_vp, C.c_int, C.c_int, _dp]),
     "d2mi_pdist2": (C.c_int, [_vp, C.c_int, C.c_size_t, C.c_size_t, _dp, _dp, _dp]),
     "d2mi_transport_batch": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _ip]),

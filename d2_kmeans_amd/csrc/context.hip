@@ -372,6 +372,88 @@ __global__ void k_f32_to_f64(const float *__restrict__ a, double *__restrict__ b
     b[i] = (double)a[i];
 }
 
+// Two-array read-modify-write streams: the access pattern of a B-ADMM plan
+// pass (Z and Y read and written in place) without its arithmetic or
+// reductions.  The arrays start at zero and stay there.
+__global__ void k_stream2_f1(float *__restrict__ a, float *__restrict__ b, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const float x = a[i], y = b[i];
+    a[i] = 0.5f * x + y;
+    b[i] = 0.5f * y;
+  }
+}
+
+__global__ void k_stream2_f4(float4 *__restrict__ a, float4 *__restrict__ b, size_t n4) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n4;
+       i += (size_t)gridDim.x * blockDim.x) {
+    float4 x = a[i], y = b[i];
+    x.x = 0.5f * x.x + y.x; x.y = 0.5f * x.y + y.y;
+    x.z = 0.5f * x.z + y.z; x.w = 0.5f * x.w + y.w;
+    y.x *= 0.5f; y.y *= 0.5f; y.z *= 0.5f; y.w *= 0.5f;
+    a[i] = x;
+    b[i] = y;
+  }
+}
+
+extern "C" int d2mi_stream_probe(d2mi_ctx *ctx, size_t n, int reps, double *gbps,
+                                 char *report, int report_len) {
+  D2MI_TRY(d2mi_check_ctx(ctx));
+  if (!gbps || n < 1024 || reps < 1) {
+    d2mi_set_error("stream probe: n >= 1024 elements and reps >= 1 required");
+    return D2MI_EINVAL;
+  }
+  n &= ~(size_t)3;  // whole float4s
+  DevMem a, b;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  double best = 0;
+  std::string rep;
+  int rc = a.ensure(n * sizeof(float));
+  if (rc == D2MI_OK) rc = b.ensure(n * sizeof(float));
+  if (rc == D2MI_OK &&
+      (hipMemsetAsync(a.p, 0, n * sizeof(float), ctx->stream) != hipSuccess ||
+       hipMemsetAsync(b.p, 0, n * sizeof(float), ctx->stream) != hipSuccess ||
+       hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+    rc = D2MI_EHIP;
+  for (int per_cu : {8, 16, 32}) {
+    if (rc != D2MI_OK) break;
+    const int grid = ctx->num_cu * per_cu;
+    for (int vec = 0; vec < 2 && rc == D2MI_OK; ++vec) {
+      float ms_best = 1e30f;
+      for (int r = 0; r <= reps; ++r) {  // r = 0 warms up
+        if (hipEventRecord(e0, ctx->stream) != hipSuccess) { rc = D2MI_EHIP; break; }
+        if (vec)
+          k_stream2_f4<<<grid, 256, 0, ctx->stream>>>(a.as<float4>(), b.as<float4>(), n / 4);
+        else
+          k_stream2_f1<<<grid, 256, 0, ctx->stream>>>(a.as<float>(), b.as<float>(), n);
+        float ms = 0;
+        if (hipEventRecord(e1, ctx->stream) != hipSuccess ||
+            hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess) { rc = D2MI_EHIP; break; }
+        if (r > 0 && ms < ms_best) ms_best = ms;
+      }
+      if (rc != D2MI_OK) break;
+      const double g = 16.0 * n / (ms_best * 1e-3) / 1e9;
+      best = g > best ? g : best;
+      char line[128];
+      snprintf(line, sizeof line, "%s %d workgroups/CU: %.4f ms, %.1f GB/s\n",
+               vec ? "float4" : "dword", per_cu, ms_best, g);
+      rep += line;
+    }
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  a.release();
+  b.release();
+  if (rc != D2MI_OK) {
+    if (rc == D2MI_EHIP) d2mi_set_error("stream probe: HIP call failed");
+    return rc;
+  }
+  *gbps = best;
+  if (report && report_len > 0) snprintf(report, report_len, "%s", rep.c_str());
+  return D2MI_OK;
+}
+
 extern "C" int d2mi_get_plan(d2mi_ctx *ctx, int phase, int precision, int which, double *out) {
   D2MI_TRY(d2mi_check_ctx(ctx));
   PhaseDev *p = d2mi_phase(ctx, phase);

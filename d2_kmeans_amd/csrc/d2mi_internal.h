@@ -54,7 +54,7 @@ struct PhaseDev {
   DevMem sym;                   // int[col] word ids / histogram bins
   DevMem vocab64, vocab32;      // [V*dim]
   DevMem vocab16, vnorm16;      // word embeddings: BF16 rows padded to d2mi_pdist2_pad(dim),
-                                // FP32 norms of the rounded rows (made on first use)
+                                // FP32 norms of the unrounded FP32 rows (made on first use)
   DevMem dist64, dist32;        // [V*V]
   // centroids (K x s)
   int K = 0, s = 0;

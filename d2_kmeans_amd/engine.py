@@ -104,6 +104,14 @@ class Engine:
         Z = np.ascontiguousarray(Z, np.float64)
         check(lib().d2mi_set_plan(self._h, idx, precision, ptr(Z)))
 
+    def stream_probe(self, n: int, reps: int = 5):
+        """(best GB/s, report) of two-array in-place FP32 streaming over n
+        elements per array: the plan kernels' practical HBM ceiling."""
+        g = C.c_double()
+        buf = C.create_string_buffer(1024)
+        check(lib().d2mi_stream_probe(self._h, int(n), int(reps), C.byref(g), buf, len(buf)))
+        return g.value, buf.value.decode()
+
     # ------------------------------------------------------------ B-ADMM
     def badmm(self, idx: int, opts: BADMMOptions, precision: int = 32,
               variant: str = "engine") -> dict:

@@ -130,6 +130,17 @@ int d2mi_get_plan(d2mi_ctx *ctx, int phase, int precision, int which, double *ou
 /* Sets the resident plan Z (double[s*col]) at `precision`. */
 int d2mi_set_plan(d2mi_ctx *ctx, int phase, int precision, const double *Z);
 
+/* Device characterisation, no counterpart in the reference: the best rate at
+ * which this GPU streams two FP32 arrays of n elements in place (each element
+ * read and written once in both arrays, 16 B per element), over dword and
+ * float4 grid-stride kernels at 8, 16 and 32 workgroups per CU, best of `reps`
+ * launches each after one warm-up.  That is the access pattern of a B-ADMM
+ * plan pass without its arithmetic, so it is the practical ceiling for the
+ * plan kernels.  Allocates 8n bytes for the call.  report (may be NULL)
+ * receives one line per variant. */
+int d2mi_stream_probe(d2mi_ctx *ctx, size_t n, int reps, double *gbps, char *report,
+                      int report_len);
+
 /* Squared Euclidean distance matrix, the reference's _dpdist2
  * (blas_like64.c:199-208, include/utils/blas_like.h:48):
  * C[i*n + j] = sum_k (A[j*d + k] - B[i*d + k])^2, A: n x d, B: m x d,

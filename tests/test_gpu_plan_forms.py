@@ -212,3 +212,17 @@ def test_prefetch_form_bit_identical(engine, n, seed):
     np.testing.assert_array_equal(ca.p_supp, cb.p_supp)
     np.testing.assert_array_equal(za, zb)
     assert sa["obj"] == sb["obj"]
+
+
+def test_stream_probe(engine):
+    """The two-array stream that bench.py reports as the plan kernels'
+    practical ceiling: six variants and a positive rate (at this size the
+    arrays sit in the last-level cache, so no HBM bound applies), and bad
+    arguments refused."""
+    from d2_kmeans_amd._lib import D2MIError
+    say("stream probe 1<<22 elements")
+    g, rep = engine.stream_probe(1 << 22, 2)
+    assert 0 < g < 1e6
+    assert len(rep.strip().splitlines()) == 6
+    with pytest.raises(D2MIError):
+        engine.stream_probe(16, 2)
