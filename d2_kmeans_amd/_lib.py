@@ -81,6 +81,7 @@ SIGNATURES = {
     "d2mi_set_plan": (C.c_int, [_vp, C.c_int, C.c_int, _dp]),
     "d2mi_pdist2": (C.c_int, [_vp, C.c_int, C.c_size_t, C.c_size_t, _dp, _dp, _dp]),
     "d2mi_transport_batch": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _ip]),
+    "d2mi_transport_profile": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _ip, _dp]),
     "d2mi_transport_batch_noalloc": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp,
                                                _ip]),
     "d2mi_pdist2_mfma": (C.c_int, [_vp, C.c_int, C.c_size_t, C.c_size_t, _dp, _dp, _dp, C.c_int,

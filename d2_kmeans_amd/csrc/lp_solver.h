@@ -52,14 +52,28 @@ constexpr int MAX_WAVES = 16;
 #endif
 // Vogel's start on reduced costs: approximate duals from entropic
 // (Sinkhorn) iterations in the log domain, the temperature lowered
-// geometrically from DUAL_EPS0 to DUAL_EPS1 times the largest cost, for
-// tables of at least DUAL_MIN_CELLS cells; more iterations for larger tables,
-// where each saved pivot costs more (measured on MI355X, 40 x 20 ... 48 x 500:
-// 16 iterations below 4096 cells, 32 below 16384, 48 beyond)
-__host__ __device__ inline int dual_iters_for(long long cells) {
-  return cells >= 16384 ? 48 : cells >= 4096 ? 32 : 16;
+// geometrically from eps0 to eps1 times the cost scale, for tables of at
+// least DUAL_MIN_CELLS cells.  The schedule depends on the table size
+// (measured on MI355X on two LP families: 40 x 20-80 image-like LPs of the
+// clustering loop, and 48 x 50-500 assignment LPs in d = 64):
+//  - below 4096 cells, 16 iterations from 0.1 to 0.002 of the largest cost.
+//    Lower final temperatures need more iterations than the pivots they
+//    save are worth, and a cooler first temperature triples the pivots of
+//    the image-like LPs, whose costs have many near-ties;
+//  - from 4096 cells on, the scale is the largest cost after row, then
+//    column minima are subtracted (the spread the LP sees), and the
+//    temperature goes from 0.1 to 2e-5 of it in 48 iterations (64 from
+//    16384 cells on): a 48 x 500 LP takes 260-310 pivots instead of 1140
+//    and 4.3 ms instead of 11.9.
+struct DualSchedule {
+  int iters;
+  float eps0, eps1;
+  int reduce;
+};
+__host__ __device__ inline DualSchedule dual_schedule(long long cells) {
+  if (cells < 4096) return {16, 0.1f, 0.002f, 0};
+  return {cells >= 16384 ? 64 : 48, 0.1f, 2e-5f, 1};
 }
-constexpr float DUAL_EPS0 = 0.1f, DUAL_EPS1 = 0.002f;
 constexpr int DUAL_MIN_CELLS = 512;
 // rows of at most DUAL_ROW_THREAD columns: one thread per row in the row
 // pass; longer rows: segments of DUAL_SEG columns, one thread each
@@ -314,6 +328,26 @@ struct Problem {
   int cpt;                // cells per thread per pricing block
   int vogel;              // start from Vogel's approximation
   int dual_iters;         // > 0: ... on reduced costs from this many entropic iterations
+  float dual_eps0;        // first temperature, relative to the cost scale
+  float dual_eps1;        // final temperature, relative to the cost scale
+  int dual_reduce;        // cost scale: 0 largest cost; 1 largest cost after
+                          // subtracting row, then column minima
+  long long *prof;        // optional: PROF_N phase times of this LP (wall-clock ticks)
+};
+
+// Phases timed when Problem::prof is set (thread 0 reads the wall clock
+// right after workgroup barriers; the pivot-loop phases are summed over
+// pivots, including the exact-cost verification rounds).
+enum ProfPhase {
+  PROF_TABLE = 0,  // cost table in LDS and the cost scale
+  PROF_DUALS,      // entropic dual iterations and the reduced-cost table
+  PROF_START,      // Vogel's approximation (or the north-west corner)
+  PROF_ROOT,       // exact table rebuilt, start cells rooted into a tree
+  PROF_JUMP,       // pointer jumping: potentials, depths, lifting table
+  PROF_PRICE,      // pricing blocks
+  PROF_PIVOT,      // apex, ratio test, flows and re-hang
+  PROF_VALUE,      // objective over the tree edges
+  PROF_N
 };
 
 // (value, index) order with the index breaking ties
@@ -649,6 +683,16 @@ __device__ __attribute__((always_inline)) double solve(const Problem &P, unsigne
   const float inv_n = 1.0f / (float)n;
   const int Tq = T / n, Tr = T - (T / n) * n;  // a step of T cells = Tq columns + Tr rows
   const int wid = tid / WAVE;
+  long long pt[PROF_N] = {};
+  long long t_last = P.prof ? (long long)wall_clock64() : 0;
+  // adds the time since the last mark to phase ph (thread 0, after a barrier)
+  auto mark = [&](int ph) {
+    if (P.prof && tid == 0) {
+      const long long t = (long long)wall_clock64();
+      pt[ph] += t - t_last;
+      t_last = t;
+    }
+  };
 
   // ---- cost table and the cost scale for the reduced-cost tolerance
   double cmax = 0.0;
@@ -678,6 +722,7 @@ __device__ __attribute__((always_inline)) double solve(const Problem &P, unsigne
   }
   cmax = block_max(cmax, ctl, bank, nw);
   const double eps = 1e-12 * (1.0 + cmax);
+  mark(PROF_TABLE);
 
   // ---- starting basis: Vogel's approximation (about half the pivots of the
   // north-west corner), else the north-west corner.  With the table in LDS,
@@ -694,11 +739,35 @@ __device__ __attribute__((always_inline)) double solve(const Problem &P, unsigne
     const bool duals = table && P.dual_iters > 1;
     if (duals) {
       float *du = (float *)(lds + L.accA), *dv = du + n;  // accA: nn doubles, idle here
-      for (int v = tid; v < nn; v += T) du[v] = 0.f;  // u and v
-      __syncthreads();
-      const float cm = (float)cmax;
+      float cm = (float)cmax;
+      if (P.dual_reduce) {
+        // u = row minima, v = column minima of c - u: the temperature is set
+        // against the spread the LP sees (c - u - v >= 0, zero in every line)
+        for (int k = tid; k < n; k += T) {
+          float mn = FLT_MAX;
+          for (int j = 0; j < m; ++j) mn = fminf(mn, ct[k + (size_t)j * n]);
+          du[k] = mn;
+        }
+        __syncthreads();
+        float rmx = 0.f;
+        for (int j = tid; j < m; j += T) {
+          const float *cc = ct + (size_t)j * n;
+          float mn = FLT_MAX, mx = -FLT_MAX;
+          for (int k = 0; k < n; ++k) {
+            const float x = cc[k] - du[k];
+            mn = fminf(mn, x);
+            mx = fmaxf(mx, x);
+          }
+          dv[j] = mn;
+          rmx = fmaxf(rmx, mx - mn);
+        }
+        cm = (float)block_max((double)rmx, ctl, bank, nw);
+      } else {
+        for (int v = tid; v < nn; v += T) du[v] = 0.f;  // u and v
+        __syncthreads();
+      }
       const int nit = P.dual_iters;
-      const float r = exp2f(log2f(DUAL_EPS1 / DUAL_EPS0) / (float)(nit - 1));
+      const float r = exp2f(log2f(P.dual_eps1 / P.dual_eps0) / (float)(nit - 1));
       // long rows: G segments of sg columns, one thread each (segment maxima
       // and sums in LDS that Vogel overwrites later: accB and flow)
       float *smx = (float *)(lds + L.accB);
@@ -706,7 +775,51 @@ __device__ __attribute__((always_inline)) double solve(const Problem &P, unsigne
       const int G = min(G0, max(1, (2 * nn) / n));  // smx, ssm: n G floats each
       const int sg = (m + G - 1) / G;
       float *ssm = smx + (size_t)n * G;
-      float te = DUAL_EPS0 * cm;
+      float te = P.dual_eps0 * cm;
+      // log-sum-exp of a line's cells x(0..len-1) in one sweep: 8-cell
+      // batches alternate between two online chains (max and sum of
+      // exp(x - max)) merged at the end, a fixed order for every line
+      // whatever the workgroup size.  Masked cells are -inf (exp gives 0).
+      const float NINF = -__builtin_huge_valf();
+      auto lse8 = [](float &mx, float &sm, const float (&x)[8]) {
+        float b = x[0];
+#pragma unroll
+        for (int u = 1; u < 8; ++u) b = fmaxf(b, x[u]);
+        const float nm = fmaxf(mx, b);
+        float acc = sm * __expf(mx - nm);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc += __expf(x[u] - nm);
+        mx = nm;
+        sm = acc;
+      };
+      auto lse_line = [&](int len, auto xf, float &mo, float &so) {
+        float ma = NINF, sa = 0.f, mb = NINF, sb = 0.f;
+        int i = 0;
+        for (; i + 16 <= len; i += 16) {
+          float xa[8], xb[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            xa[u] = xf(i + u);
+            xb[u] = xf(i + 8 + u);
+          }
+          lse8(ma, sa, xa);
+          lse8(mb, sb, xb);
+        }
+        if (i < len) {  // 1..15 cells left
+          float xa[8], xb[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            const int ia = i + u, ib = i + 8 + u;
+            const float va = xf(min(ia, len - 1)), vb = xf(min(ib, len - 1));
+            xa[u] = ia < len ? va : NINF;
+            xb[u] = ib < len ? vb : NINF;
+          }
+          lse8(ma, sa, xa);  // holds a cell: ma is finite from here on
+          if (i + 8 < len) lse8(mb, sb, xb);
+        }
+        mo = fmaxf(ma, mb);
+        so = sa * __expf(ma - mo) + (sb > 0.f ? sb * __expf(mb - mo) : 0.f);
+      };
       for (int it = 0; it < nit && cm > 1e-30f; ++it, te *= r) {
         const float ie = 1.0f / te;
         // v_j = te (log b_j - LSE_k (u_k - c_kj) / te): one thread per
@@ -714,72 +827,40 @@ __device__ __attribute__((always_inline)) double solve(const Problem &P, unsigne
         // the table; the rotation spreads the threads over the LDS banks)
         for (int j = tid; j < m; j += T) {
           const float *cc = ct + (size_t)j * n;
-          const int k0 = j % n;
-          float mx = -FLT_MAX;
-          for (int i0 = 0; i0 < n; i0 += 8) {
-            float x[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-              int k = k0 + i0 + u;
-              k = min(k >= n ? k - n : k, n - 1);  // past the column: any cell, masked
-              x[u] = i0 + u < n ? (du[k] - cc[k]) * ie : -FLT_MAX;
-            }
-#pragma unroll
-            for (int u = 0; u < 8; ++u) mx = fmaxf(mx, x[u]);
-          }
-          float sm = 0.f;
-          for (int i0 = 0; i0 < n; i0 += 8) {
-            float x[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-              int k = k0 + i0 + u;
-              k = min(k >= n ? k - n : k, n - 1);  // past the column: any cell, masked
-              x[u] = i0 + u < n ? (du[k] - cc[k]) * ie : -FLT_MAX;
-            }
-#pragma unroll
-            for (int u = 0; u < 8; ++u) sm += i0 + u < n ? __expf(x[u] - mx) : 0.f;
-          }
+          const int k1 = j % n;
+          float mx, sm;
+          lse_line(n, [&](int i) {
+            int k = k1 + i;
+            k = k >= n ? k - n : k;
+            return (du[k] - cc[k]) * ie;
+          }, mx, sm);
           dv[j] = te * (__logf(fmaxf((float)P.wb[j], 1e-30f)) - (mx + __logf(sm)));
         }
         __syncthreads();
         // u_k = te (log a_k - LSE_j (v_j - c_kj) / te): row k's segment g
         // by one thread (consecutive threads, consecutive rows: consecutive
-        // addresses), then the segments combined in order
-        auto seg_pass = [&](bool sum_pass) {
-          for (int w = tid; w < n * G; w += T) {
-            const int k = w % n, g = w / n;
-            const int j0 = g * sg, j1 = min(m, j0 + sg);
-            float rm = 0.f;
-            if (sum_pass) {
-              rm = -FLT_MAX;
-              for (int h = 0; h < G; ++h) rm = fmaxf(rm, smx[k + h * n]);
-            }
-            float acc = sum_pass ? 0.f : -FLT_MAX;
-            for (int j = j0; j < j1; j += 8) {
-              float x[8];
-#pragma unroll
-              for (int u = 0; u < 8; ++u) {
-                const int jj = min(j + u, j1 - 1);
-                x[u] = (dv[jj] - ct[k + (size_t)jj * n]) * ie;
-              }
-#pragma unroll
-              for (int u = 0; u < 8; ++u) {
-                if (sum_pass) acc += j + u < j1 ? __expf(x[u] - rm) : 0.f;
-                else acc = fmaxf(acc, x[u]);
-              }
-            }
-            if (sum_pass) ssm[w] = acc;
-            else smx[w] = acc;
-          }
-        };
-        seg_pass(false);
-        if (G > 1) __syncthreads();
-        seg_pass(true);
+        // addresses); then the segments combined in order
+        for (int w = tid; w < n * G; w += T) {
+          const int k = w % n, g = w / n;
+          const int j0 = g * sg, len = max(0, min(m, j0 + sg) - j0);  // 0: an empty segment
+          float mx = NINF, sm = 0.f;
+          if (len > 0)
+            lse_line(len, [&](int i) {
+              const int jj = j0 + i;
+              return (dv[jj] - ct[k + (size_t)jj * n]) * ie;
+            }, mx, sm);
+          smx[w] = mx;
+          ssm[w] = sm;
+        }
         if (G > 1) __syncthreads();
         for (int k = tid; k < n; k += T) {
-          float rm = -FLT_MAX, sm = 0.f;
+          float rm = -FLT_MAX;
           for (int h = 0; h < G; ++h) rm = fmaxf(rm, smx[k + h * n]);
-          for (int h = 0; h < G; ++h) sm += ssm[k + h * n];
+          float sm = 0.f;
+          for (int h = 0; h < G; ++h) {
+            const float sh = ssm[k + h * n];
+            sm += sh > 0.f ? sh * __expf(smx[k + h * n] - rm) : 0.f;
+          }
           du[k] = te * (__logf(fmaxf((float)P.wa[k], 1e-30f)) - (rm + __logf(sm)));
         }
         __syncthreads();
@@ -790,10 +871,12 @@ __device__ __attribute__((always_inline)) double solve(const Problem &P, unsigne
         ct[e] = ct[e] - du[k] - dv[j];
       }
       __syncthreads();
+      mark(PROF_DUALS);
     }
     if (table) vogel_start<true>(P, lds, L, ct);
     else vogel_start<false>(P, lds, L, ct);
     __syncthreads();
+    mark(PROF_START);
     if (duals) {  // the exact FP32 table again, as built above
       for (int e = tid; e < N; e += T) {
         if (P.table == TAB_GLOBAL32) {
@@ -826,6 +909,7 @@ __device__ __attribute__((always_inline)) double solve(const Problem &P, unsigne
     int missing = 0;
     for (int v = tid; v < nn; v += T) missing |= par[v] == -2;
     have_start = !block_or(missing, ctl, bank, nw);
+    mark(PROF_ROOT);
   }
   if (!have_start) {
     // north-west corner: each new cell brings one new node, hung under the
@@ -853,6 +937,7 @@ __device__ __attribute__((always_inline)) double solve(const Problem &P, unsigne
       }
     }
     __syncthreads();
+    mark(PROF_START);
   }
 
   // cost of the tree edge at node v (edge to parent p), table or exact
@@ -985,6 +1070,7 @@ __device__ __attribute__((always_inline)) double solve(const Problem &P, unsigne
       dep = in_b ? depB : depA;
     }
     double *pfl = pot == accA ? accB : accA;  // free during the pivot
+    mark(PROF_JUMP);  // after the last round's barrier (solo: the one above)
 
     // ---- pricing
     const bool bland = pivots >= dantzig_limit;
@@ -1088,6 +1174,7 @@ __device__ __attribute__((always_inline)) double solve(const Problem &P, unsigne
       }
     }
     (void)ent_r;
+    mark(PROF_PRICE);
     if (ent == INT_MAX) {
       if (!exact_mode) {  // optimal for the FP32 table: verify on exact costs
         exact_mode = true;
@@ -1235,6 +1322,7 @@ __device__ __attribute__((always_inline)) double solve(const Problem &P, unsigne
       }
     }
     __syncthreads();
+    mark(PROF_PIVOT);
     ++pivots;
   }
 
@@ -1252,6 +1340,9 @@ __device__ __attribute__((always_inline)) double solve(const Problem &P, unsigne
   }
   const double val = sum[0];
   if (pivots_out) *pivots_out = pivots;
+  mark(PROF_VALUE);
+  if (P.prof && tid == 0)
+    for (int i = 0; i < PROF_N; ++i) P.prof[i] = pt[i];
   __syncthreads();
   return val;
 }

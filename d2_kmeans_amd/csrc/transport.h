@@ -47,13 +47,22 @@ struct LpArgs {
   int n_max, m_max;
   // launch shape, filled by d2mi_plan_transport
   int threads, cpt, table, stage_a, stage_b, vogel;
-  int dual_iters;  // entropic iterations for Vogel's reduced-cost start (0: plain start)
+  // Vogel's reduced-cost start: each LP takes lp::dual_schedule(n * m) of its
+  // own size (so its result does not depend on the other LPs of the launch)
+  // when dual_ok and n * m >= dual_min; the fields below override the
+  // schedule when >= 0 (dual_iters 0: plain Vogel)
+  int dual_ok;
+  long long dual_min;
+  int dual_iters;
+  float dual_eps0, dual_eps1;
+  int dual_reduce;
   size_t lds;
   int threads_hint;     // > 0: workgroup size to use
   hipStream_t stream;   // null: the context stream
   double *out;
   int *pivots;  // may be null
   int *status;  // bit 0: an LP hit the pivot limit
+  long long *prof;  // optional: lp::PROF_N phase times per job (wall-clock ticks)
 };
 
 int d2mi_plan_transport(d2mi_ctx *ctx, LpArgs &a);

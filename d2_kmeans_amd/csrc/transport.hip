@@ -12,7 +12,12 @@
 
 namespace {
 
-__global__ __launch_bounds__(1024) void k_transport(LpArgs a) {
+// One LP per workgroup (a grid-stride loop over jobs).  MAXT bounds the
+// workgroup size the instantiation accepts: a smaller bound lets the
+// compiler keep the solver's state in registers (at 1024 threads it has 128
+// VGPRs per lane and spills).
+template <int MAXT>
+__global__ __launch_bounds__(MAXT) void k_transport(LpArgs a) {
   extern __shared__ __align__(16) unsigned char lds[];
   const int tid = threadIdx.x, T = blockDim.x;
   // rows carry supports for Euclidean and word phases; columns for Euclidean
@@ -86,7 +91,16 @@ __global__ __launch_bounds__(1024) void k_transport(LpArgs a) {
     P.tab_rel = 0x1p-22f;
     P.cpt = a.cpt;
     P.vogel = a.vogel;
-    P.dual_iters = a.dual_iters;
+    {
+      const long long cells = (long long)n * m;
+      const lp::DualSchedule ds = lp::dual_schedule(cells);
+      const int it = a.dual_iters >= 0 ? a.dual_iters : ds.iters;
+      P.dual_iters = a.dual_ok && cells >= a.dual_min && it >= 2 ? it : 0;
+      P.dual_eps0 = a.dual_eps0 > 0.f ? a.dual_eps0 : ds.eps0;
+      P.dual_eps1 = a.dual_eps1 > 0.f ? a.dual_eps1 : ds.eps1;
+      P.dual_reduce = a.dual_reduce >= 0 ? a.dual_reduce : ds.reduce;
+    }
+    P.prof = a.prof ? a.prof + (size_t)job * lp::PROF_N : nullptr;
     int piv = 0;
     const double v = lp::solve(P, lds, L, &piv, a.status);
     if (tid == 0) {
@@ -157,6 +171,11 @@ static int env_int(const char *name, int dflt) {
   return v && *v ? atoi(v) : dflt;
 }
 
+static double env_double(const char *name, double dflt) {
+  const char *v = getenv(name);
+  return v && *v ? atof(v) : dflt;
+}
+
 int d2mi_plan_transport(d2mi_ctx *ctx, LpArgs &a) {
   const bool word = a.cost_mode == LP_COST_WORD;
   const bool euclid = a.cost_mode == LP_COST_EUCLID || (word && a.b_is_table);
@@ -207,11 +226,15 @@ int d2mi_plan_transport(d2mi_ctx *ctx, LpArgs &a) {
   const bool hist = a.cost_mode == LP_COST_HIST || a.cost_mode == LP_COST_SPARSE;
   a.vogel = env_int("D2MI_LP_VOGEL", hist ? 0 : 1) != 0;
   // Vogel on reduced costs from approximate duals, where the pivots it
-  // saves outweigh its entropic iterations (D2MI_LP_DUAL_MIN: cell threshold,
-  // D2MI_LP_DUAL_ITERS: iteration count, 0 = plain Vogel)
-  const long long dual_min = env_int("D2MI_LP_DUAL_MIN", lp::DUAL_MIN_CELLS);
-  const int dual_iters = env_int("D2MI_LP_DUAL_ITERS", lp::dual_iters_for((long long)cells));
-  a.dual_iters = a.vogel && tab && (long long)cells >= dual_min && dual_iters >= 2 ? dual_iters : 0;
+  // saves outweigh its entropic iterations.  Overrides for tuning:
+  // D2MI_LP_DUAL_MIN (cell threshold), D2MI_LP_DUAL_ITERS (0 = plain Vogel),
+  // D2MI_LP_DUAL_EPS0 / _EPS1 (first and last temperature), D2MI_LP_DUAL_REDUCE
+  a.dual_ok = a.vogel && tab;
+  a.dual_min = env_int("D2MI_LP_DUAL_MIN", lp::DUAL_MIN_CELLS);
+  a.dual_iters = env_int("D2MI_LP_DUAL_ITERS", -1);
+  a.dual_eps0 = (float)env_double("D2MI_LP_DUAL_EPS0", -1.0);
+  a.dual_eps1 = (float)env_double("D2MI_LP_DUAL_EPS1", -1.0);
+  a.dual_reduce = env_int("D2MI_LP_DUAL_REDUCE", -1);
   return D2MI_OK;
 }
 
@@ -219,10 +242,16 @@ int d2mi_launch_transport(d2mi_ctx *ctx, const LpArgs &a0) {
   if (a0.njobs <= 0) return D2MI_OK;
   LpArgs a = a0;
   D2MI_TRY(d2mi_plan_transport(ctx, a));
+  // the instantiation with the smallest bound that admits the workgroup
+  // (D2MI_LP_BOUND overrides it: 256, 512 or 1024)
+  const int want = env_int("D2MI_LP_BOUND", 0);
+  const int bound = want >= a.threads && (want == 256 || want == 512 || want == 1024) ? want
+                    : a.threads <= 256 ? 256 : a.threads <= 512 ? 512 : 1024;
+  auto kern = bound == 256 ? k_transport<256> : bound == 512 ? k_transport<512> : k_transport<1024>;
   int per_cu = 0;
-  D2MI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_transport, a.threads, a.lds));
+  D2MI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, a.threads, a.lds));
   const int grid = std::min(a.njobs, ctx->num_cu * std::max(1, per_cu));
-  k_transport<<<grid, a.threads, a.lds, a.stream ? a.stream : ctx->stream>>>(a);
+  kern<<<grid, a.threads, a.lds, a.stream ? a.stream : ctx->stream>>>(a);
   D2MI_HIP(hipGetLastError());
   return D2MI_OK;
 }
@@ -245,59 +274,102 @@ int d2mi_sqrt_clamp(d2mi_ctx *ctx, double *p, int n) {
 
 // --------------------------------------------------------------- C ABI
 
-extern "C" int d2mi_transport_batch(d2mi_ctx *ctx, int n, int m, int nbatch, const double *C,
-                                    const double *wX, const double *wY, double *out,
-                                    int *pivots) {
+namespace {
+// Device staging of one d2mi_transport_batch call, released on every return.
+struct BatchStaging {
+  DevMem C, A, B, out, ja, jb, piv, st, prof;
+  ~BatchStaging() {
+    for (DevMem *m : {&C, &A, &B, &out, &ja, &jb, &piv, &st, &prof}) m->release();
+  }
+};
+
+// d2mi_transport_batch, and with prof_us the per-LP phase times
+// (lp::PROF_N per LP, microseconds) of d2mi_transport_profile.
+int transport_batch(d2mi_ctx *ctx, int n, int m, int nbatch, const double *C, const double *wX,
+                    const double *wY, double *out, int *pivots, double *prof_us) {
   D2MI_TRY(d2mi_check_ctx(ctx));
   if (n <= 0 || m <= 0 || nbatch <= 0 || !C || !wX || !wY || !out) {
     d2mi_set_error("d2mi_transport_batch: bad arguments");
     return D2MI_EINVAL;
   }
   hipStream_t S = ctx->stream;
-  DevMem dC, dA, dB, dout, dja, djb, dpiv, dst;
+  BatchStaging d;
   const size_t nm = (size_t)n * m;
-  D2MI_TRY(dC.assign(C, sizeof(double) * nm * nbatch, S));
-  D2MI_TRY(dA.assign(wX, sizeof(double) * (size_t)n * nbatch, S));
-  D2MI_TRY(dB.assign(wY, sizeof(double) * (size_t)m * nbatch, S));
-  D2MI_TRY(dout.ensure(sizeof(double) * nbatch));
-  D2MI_TRY(dpiv.ensure(sizeof(int) * nbatch));
-  D2MI_TRY(dst.ensure(sizeof(int)));
+  D2MI_TRY(d.C.assign(C, sizeof(double) * nm * nbatch, S));
+  D2MI_TRY(d.A.assign(wX, sizeof(double) * (size_t)n * nbatch, S));
+  D2MI_TRY(d.B.assign(wY, sizeof(double) * (size_t)m * nbatch, S));
+  D2MI_TRY(d.out.ensure(sizeof(double) * nbatch));
+  D2MI_TRY(d.piv.ensure(sizeof(int) * nbatch));
+  D2MI_TRY(d.st.ensure(sizeof(int)));
   std::vector<int> idx(nbatch);
   for (int i = 0; i < nbatch; ++i) idx[i] = i;
-  D2MI_TRY(dja.assign(idx.data(), sizeof(int) * nbatch, S));
-  D2MI_TRY(djb.assign(idx.data(), sizeof(int) * nbatch, S));
-  D2MI_TRY(d2mi_zero_doubles(ctx, dout.as<double>(), nbatch));
-  D2MI_TRY(d2mi_zero_ints(ctx, dpiv.as<int>(), nbatch));
-  D2MI_TRY(d2mi_zero_ints(ctx, dst.as<int>(), 1));
+  D2MI_TRY(d.ja.assign(idx.data(), sizeof(int) * nbatch, S));
+  D2MI_TRY(d.jb.assign(idx.data(), sizeof(int) * nbatch, S));
+  D2MI_TRY(d2mi_zero_doubles(ctx, d.out.as<double>(), nbatch));
+  D2MI_TRY(d2mi_zero_ints(ctx, d.piv.as<int>(), nbatch));
+  D2MI_TRY(d2mi_zero_ints(ctx, d.st.as<int>(), 1));
+  const int np = nbatch * lp::PROF_N;
+  if (prof_us) {
+    D2MI_TRY(d.prof.ensure(sizeof(long long) * np));
+    D2MI_TRY(d2mi_zero_doubles(ctx, d.prof.as<double>(), np));  // all-zero bits
+  }
   LpArgs a{};
   a.njobs = nbatch;
-  a.job_a = dja.as<int>();
-  a.job_b = djb.as<int>();
+  a.job_a = d.ja.as<int>();
+  a.job_b = d.jb.as<int>();
   a.b_is_table = 1;
   a.cost_mode = LP_COST_GLOBAL;
   a.sA = n;
-  a.A_w = dA.as<double>();
+  a.A_w = d.A.as<double>();
   a.sB = m;
-  a.B_w = dB.as<double>();
-  a.gcost = dC.as<double>();
+  a.B_w = d.B.as<double>();
+  a.gcost = d.C.as<double>();
   a.gstride = (long long)nm;
   a.n_max = n;
   a.m_max = m;
-  a.out = dout.as<double>();
-  a.pivots = dpiv.as<int>();
-  a.status = dst.as<int>();
+  a.out = d.out.as<double>();
+  a.pivots = d.piv.as<int>();
+  a.status = d.st.as<int>();
+  a.prof = prof_us ? d.prof.as<long long>() : nullptr;
   D2MI_TRY(d2mi_launch_transport(ctx, a));
-  D2MI_HIP(hipMemcpyAsync(out, dout.p, sizeof(double) * nbatch, hipMemcpyDeviceToHost, S));
+  D2MI_HIP(hipMemcpyAsync(out, d.out.p, sizeof(double) * nbatch, hipMemcpyDeviceToHost, S));
   if (pivots)
-    D2MI_HIP(hipMemcpyAsync(pivots, dpiv.p, sizeof(int) * nbatch, hipMemcpyDeviceToHost, S));
+    D2MI_HIP(hipMemcpyAsync(pivots, d.piv.p, sizeof(int) * nbatch, hipMemcpyDeviceToHost, S));
+  std::vector<long long> ticks(prof_us ? np : 0);
+  if (prof_us)
+    D2MI_HIP(hipMemcpyAsync(ticks.data(), d.prof.p, sizeof(long long) * np,
+                            hipMemcpyDeviceToHost, S));
   int st = 0;
-  D2MI_HIP(hipMemcpyAsync(&st, dst.p, sizeof(int), hipMemcpyDeviceToHost, S));
+  D2MI_HIP(hipMemcpyAsync(&st, d.st.p, sizeof(int), hipMemcpyDeviceToHost, S));
   D2MI_HIP(hipStreamSynchronize(S));
+  if (prof_us) {
+    int khz = 0;  // the wall clock's rate
+    D2MI_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->device));
+    const double us_per_tick = khz > 0 ? 1e3 / khz : 0.0;
+    for (int i = 0; i < np; ++i) prof_us[i] = ticks[i] * us_per_tick;
+  }
   if (st) {
     d2mi_set_error("a transport LP hit the pivot limit");
     return D2MI_ENUMERIC;
   }
   return D2MI_OK;
+}
+}  // namespace
+
+extern "C" int d2mi_transport_batch(d2mi_ctx *ctx, int n, int m, int nbatch, const double *C,
+                                    const double *wX, const double *wY, double *out,
+                                    int *pivots) {
+  return transport_batch(ctx, n, m, nbatch, C, wX, wY, out, pivots, nullptr);
+}
+
+extern "C" int d2mi_transport_profile(d2mi_ctx *ctx, int n, int m, int nbatch, const double *C,
+                                      const double *wX, const double *wY, double *out,
+                                      int *pivots, double *phase_us) {
+  if (!phase_us) {
+    d2mi_set_error("d2mi_transport_profile: null phase_us");
+    return D2MI_EINVAL;
+  }
+  return transport_batch(ctx, n, m, nbatch, C, wX, wY, out, pivots, phase_us);
 }
 
 extern "C" int d2mi_transport_batch_noalloc(d2mi_ctx *ctx, int n, int m, int nbatch,

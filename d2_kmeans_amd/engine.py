@@ -191,6 +191,23 @@ class Engine:
                                          ptr(piv, _lib._ip)))
         return (out, piv) if pivots else out
 
+    LP_PHASES = ("table", "duals", "start", "root", "jump", "price", "pivot", "value")
+
+    def transport_profile(self, C, wX, wY):
+        """transport_batch plus each LP's time per solver phase:
+        (values, pivots, phase_us of shape (B, 8) in the order LP_PHASES)."""
+        wX = np.ascontiguousarray(np.atleast_2d(wX), np.float64)
+        wY = np.ascontiguousarray(np.atleast_2d(wY), np.float64)
+        nb, n = wX.shape
+        m = wY.shape[1]
+        C = np.ascontiguousarray(C, np.float64).reshape(nb, n * m)
+        out = np.empty(nb)
+        piv = np.empty(nb, np.int32)
+        ph = np.empty((nb, len(self.LP_PHASES)))
+        check(lib().d2mi_transport_profile(self._h, n, m, nb, ptr(C), ptr(wX), ptr(wY), ptr(out),
+                                           ptr(piv, _lib._ip), ptr(ph)))
+        return out, piv, ph
+
     # -------------------------------------------------------- assignment
     def distances(self, obj, cent, selected_phase: int = -1) -> np.ndarray:
         """Batched d2_compute_distance (clustering.c:33-128): exact

@@ -74,3 +74,27 @@ def test_lp_degenerate_grid_costs(engine):
     for b in range(nb):
         ref = O.transport(Cs[b], wXs[b], wYs[b])
         assert abs(got[b] - ref) <= 1e-10 * max(1.0, abs(ref))
+
+
+@pytest.mark.parametrize("n,m,nb", [(40, 50, 8), (48, 500, 2), (3, 4, 4)])
+def test_lp_phase_profile(engine, n, m, nb):
+    """d2mi_transport_profile: the same values and pivots as the plain batch,
+    every phase time non-negative, the pivot-loop phases present whenever an
+    LP pivoted, and the total within the call's wall time."""
+    import time
+    print(f"transport profile n={n} m={m} nb={nb}", flush=True)
+    rng = np.random.default_rng(7 + n + m)
+    A = rng.standard_normal((nb, n, 8))
+    B = rng.standard_normal((nb, m, 8))
+    C = ((A[:, :, None, :] - B[:, None, :, :]) ** 2).sum(-1)
+    Cf = np.ascontiguousarray(C.transpose(0, 2, 1)).reshape(nb, -1)
+    wX = rng.dirichlet(np.ones(n), size=nb)
+    wY = rng.dirichlet(np.ones(m), size=nb)
+    v0, p0 = engine.transport_batch(Cf, wX, wY, pivots=True)
+    t0 = time.perf_counter()
+    v1, p1, ph = engine.transport_profile(Cf, wX, wY)
+    wall_us = (time.perf_counter() - t0) * 1e6
+    assert np.array_equal(v0, v1) and np.array_equal(p0, p1)
+    assert ph.shape == (nb, 8) and (ph >= 0).all()
+    assert (ph.sum(1) > 0).all() and (ph.sum(1) < wall_us).all()
+    assert (ph[p1 > 0][:, 4:7].sum(1) > 0).all()
