@@ -260,6 +260,38 @@ def assign_leg(eng, args):
                          "dist_max_rel_err": float(np.max(np.abs(do - dist[:nchk]) /
                                                           np.maximum(do, 1e-300)))},
     }
+    # latency by query size, and where one exact LP of each size spends its
+    # time (d2mi_transport_profile: one LP per launch, a query against the
+    # centroid nearest by mean, as the server's first round would pick)
+    m_of = qph.p_str
+    res["by_size"] = {}
+    for lo_m, hi_m in ((5, 64), (65, 192), (193, 500)):
+        sel = (m_of >= lo_m) & (m_of <= hi_m)
+        if sel.any():
+            res["by_size"][f"m{lo_m}-{hi_m}"] = {
+                "queries": int(sel.sum()), "p50_ms": float(np.percentile(lat[sel], 50)),
+                "p99_ms": float(np.percentile(lat[sel], 99))}
+    cw = cent.p_w.reshape(K, s)
+    cs = cent.p_supp.reshape(K, s, d)
+    mu_c = cs.mean(1)
+    res["lp_phases"] = {}
+    for lo_m, hi_m in ((45, 55), (190, 210), (450, 500)):
+        idx = [i for i in range(qph.size) if lo_m <= m_of[i] <= hi_m][:8]
+        if not idx:
+            continue
+        rows = []
+        for i in idx:
+            w, x = Q[i]
+            k = int(np.argmin(((mu_c - x.mean(0)) ** 2).sum(1)))
+            C = ((cs[k][:, None, :] - x[None, :, :]) ** 2).sum(-1)  # (i, j) at i + j*s
+            _, piv, ph_us = eng.transport_profile(C.T.reshape(1, -1), cw[k][None], w[None])
+            rows.append((int(piv[0]), ph_us[0]))
+        P = np.array([r[1] for r in rows])
+        pv = np.array([r[0] for r in rows])
+        res["lp_phases"][f"{s}x{lo_m}-{hi_m}"] = {
+            "lps": len(rows), "median_us": float(np.median(P.sum(1))),
+            "median_pivots": float(np.median(pv)),
+            "phase_median_us": {nm: float(x) for nm, x in zip(eng.LP_PHASES, np.median(P, 0))}}
     log(f"[assign] {res}")
     return res
 
