@@ -289,7 +289,7 @@ def assign_leg(eng, args):
         P = np.array([r[1] for r in rows])
         pv = np.array([r[0] for r in rows])
         res["lp_phases"][f"{s}x{lo_m}-{hi_m}"] = {
-            "lps": len(rows), "median_us": float(np.median(P.sum(1))),
+            "lps": len(rows), "median_us": float(np.median(P[:, :8].sum(1))),
             "median_pivots": float(np.median(pv)),
             "phase_median_us": {nm: float(x) for nm, x in zip(eng.LP_PHASES, np.median(P, 0))}}
     log(f"[assign] {res}")

@@ -347,6 +347,11 @@ enum ProfPhase {
   PROF_PRICE,      // pricing blocks
   PROF_PIVOT,      // apex, ratio test, flows and re-hang
   PROF_VALUE,      // objective over the tree edges
+  // inside PROF_START (timed by lane 0 of the wave that runs Vogel's steps)
+  PROF_V_ARGMAX,   // the line with the largest penalty
+  PROF_V_TAKE,     // its cell: quantity, crossing out
+  PROF_V_SCAN,     // the lines whose two cheapest cells were crossed out
+  PROF_V_WALK,     // their top-2 minima again
   PROF_N
 };
 
@@ -582,6 +587,16 @@ __device__ __attribute__((always_inline)) void vogel_start(const Problem &P, uns
   for (int j = lane; j < m; j += WAVE) col_top2(j);
   __builtin_amdgcn_wave_barrier();
   int nr = n, nc = m;
+  // sub-phase times (lane 0; added to P.prof by solve's thread 0, which is this lane)
+  long long vt[4] = {0, 0, 0, 0};
+  long long vlast = P.prof ? (long long)wall_clock64() : 0;
+  auto vmark = [&](int i) {
+    if (P.prof && lane == 0) {
+      const long long tnow = (long long)wall_clock64();
+      vt[i] += tnow - vlast;
+      vlast = tnow;
+    }
+  };
   for (int t = 0; t < nn - 1; ++t) {
     // line with the largest penalty (crossed-out lines hold -2)
     float bp = -1.0f;
@@ -615,6 +630,7 @@ __device__ __attribute__((always_inline)) void vogel_start(const Problem &P, uns
     pen_merge(dpp_f<DPP_BC31, 0xc>(bp, -FLT_MAX), dpp_i<DPP_BC31, 0xc>(bl, INT_MAX));
     bp = lane63_f(bp);
     bl = lane63_i(bl);
+    vmark(0);
     const int is = bl < n ? bl : ma1[bl];
     const int js = bl < n ? ma1[bl] : bl - n;
     const double ra = rem[is], rb = rem[n + js];
@@ -633,6 +649,7 @@ __device__ __attribute__((always_inline)) void vogel_start(const Problem &P, uns
       pen[gone] = -2.0f;
     }
     __builtin_amdgcn_wave_barrier();
+    vmark(1);
     if (t == nn - 2) break;
     int cnt = 0;
     if (cross_row) {
@@ -644,6 +661,7 @@ __device__ __attribute__((always_inline)) void vogel_start(const Problem &P, uns
         cnt = append(aff, j, cnt);
       }
       __builtin_amdgcn_wave_barrier();
+      vmark(2);
       lines_top2(std::true_type{}, cnt);
     } else {
       --nc;
@@ -654,12 +672,16 @@ __device__ __attribute__((always_inline)) void vogel_start(const Problem &P, uns
         cnt = append(aff, k, cnt);
       }
       __builtin_amdgcn_wave_barrier();
+      vmark(2);
       // one row: the whole wave if it is long; else every row at once
       if (cnt == 1 && m >= 128) row_top2_wave(redo[0]);
       else lines_top2(std::false_type{}, cnt);
     }
     __builtin_amdgcn_wave_barrier();
+    vmark(3);
   }
+  if (P.prof && lane == 0)
+    for (int i = 0; i < 4; ++i) P.prof[PROF_V_ARGMAX + i] = vt[i];
 }
 
 // Solves one LP with the whole workgroup (blockDim.x a multiple of 64, at
@@ -1342,7 +1364,7 @@ __device__ __attribute__((always_inline)) double solve(const Problem &P, unsigne
   if (pivots_out) *pivots_out = pivots;
   mark(PROF_VALUE);
   if (P.prof && tid == 0)
-    for (int i = 0; i < PROF_N; ++i) P.prof[i] = pt[i];
+    for (int i = 0; i <= PROF_VALUE; ++i) P.prof[i] = pt[i];
   __syncthreads();
   return val;
 }

@@ -191,11 +191,13 @@ class Engine:
                                          ptr(piv, _lib._ip)))
         return (out, piv) if pivots else out
 
-    LP_PHASES = ("table", "duals", "start", "root", "jump", "price", "pivot", "value")
+    LP_PHASES = ("table", "duals", "start", "root", "jump", "price", "pivot", "value",
+                 "start_argmax", "start_take", "start_scan", "start_walk")
 
     def transport_profile(self, C, wX, wY):
         """transport_batch plus each LP's time per solver phase:
-        (values, pivots, phase_us of shape (B, 8) in the order LP_PHASES)."""
+        (values, pivots, phase_us of shape (B, 12) in the order LP_PHASES;
+        the last four split the "start" phase)."""
         wX = np.ascontiguousarray(np.atleast_2d(wX), np.float64)
         wY = np.ascontiguousarray(np.atleast_2d(wY), np.float64)
         nb, n = wX.shape

@@ -159,14 +159,17 @@ int d2mi_transport_batch(d2mi_ctx *ctx, int n, int m, int nbatch, const double *
                          const double *wX, const double *wY, double *out, int *pivots);
 
 /* Phases of one transport LP, in the order d2mi_transport_profile reports them. */
-#define D2MI_LP_PHASES 8
+#define D2MI_LP_PHASES 12
 /* d2mi_transport_batch that also reports where each LP spent its time; no
- * counterpart in the reference (Mosek is closed).  phase_us[b*8 + p] (host,
- * nbatch*8 doubles) receives LP b's wall time in microseconds in phase p:
+ * counterpart in the reference (Mosek is closed).  phase_us[b*12 + p] (host,
+ * nbatch*12 doubles) receives LP b's wall time in microseconds in phase p:
  * 0 cost table, 1 entropic duals, 2 starting basis (Vogel or north-west),
  * 3 start rooted into a tree, 4 pointer jumping (summed over pivots),
  * 5 pricing (summed), 6 pivots (apex, ratio test, flows, re-hang; summed),
- * 7 objective.  Values and pivots equal d2mi_transport_batch's. */
+ * 7 objective; and within phase 2, Vogel's steps split into 8 finding the
+ * line of largest penalty, 9 taking its cell, 10 finding the lines whose
+ * two cheapest cells were crossed out, 11 walking them again.  Values and
+ * pivots equal d2mi_transport_batch's. */
 int d2mi_transport_profile(d2mi_ctx *ctx, int n, int m, int nbatch, const double *C,
                            const double *wX, const double *wY, double *out, int *pivots,
                            double *phase_us);

@@ -95,6 +95,7 @@ def test_lp_phase_profile(engine, n, m, nb):
     v1, p1, ph = engine.transport_profile(Cf, wX, wY)
     wall_us = (time.perf_counter() - t0) * 1e6
     assert np.array_equal(v0, v1) and np.array_equal(p0, p1)
-    assert ph.shape == (nb, 8) and (ph >= 0).all()
-    assert (ph.sum(1) > 0).all() and (ph.sum(1) < wall_us).all()
+    assert ph.shape == (nb, 12) and (ph >= 0).all()
+    assert (ph[:, :8].sum(1) > 0).all() and (ph[:, :8].sum(1) < wall_us).all()
+    assert (ph[:, 8:].sum(1) <= ph[:, 2] + 1.0).all()  # Vogel's steps lie inside the start
     assert (ph[p1 > 0][:, 4:7].sum(1) > 0).all()
