@@ -141,6 +141,7 @@ template <typename T>
 struct PlanArgs {
   int N, s, d, K, sp, m_max, chunk, R, V;
   int upd, resid, keepx, first, force_init;
+  int rev;  // register forms: walk each chunk's objects last to first
   const char *lsw;
   const int *__restrict__ p_str;
   const long long *__restrict__ cum;
@@ -1039,9 +1040,13 @@ void k_badmm_plan_reg(PlanArgs<T> a) {
       }
     }
   };
-  if (PF && p0 < p1) prefetch(p0);
+  // odd passes walk the chunk backwards, so a pass starts on the objects the
+  // previous pass wrote last, which are still in the memory-side cache
+  const int np = p1 - p0, rev = a.rev;
+  if (PF && np > 0) prefetch(rev ? p1 - 1 : p0);
 
-  for (int p = p0; p < p1; ++p) {
+  for (int i = 0; i < np; ++i) {
+    const int p = rev ? p1 - 1 - i : p0 + i;
     // per-object copies of the thread's row and first column: without this
     // the compiler keeps every column slot's addresses live across objects
     int ke = ke_t, jb = jb_t;
@@ -1188,7 +1193,7 @@ void k_badmm_plan_reg(PlanArgs<T> a) {
     }
     __syncthreads();
     // t (in stZ) is read: the next object's copy can start
-    if (PF && p + 1 < p1) prefetch(p + 1);
+    if (PF && i + 1 < np) prefetch(rev ? p - 1 : p + 1);
     // pass 3 -- step 1c: x = t / colsum * w (:159); step 2a: z' = x exp(Y) +
     // 1e-9 (:166); partial row sums (_drnorm, :169) and step 5 partials
     {
@@ -2795,6 +2800,9 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
     const bool force_gm = getenv_flag("D2MI_BADMM_HBM_SCRATCH");
     const bool no_64 = getenv_flag("D2MI_PLAN_LEGACY"), no_reg = getenv_flag("D2MI_PLAN_NOREG");
     const bool no_lean = getenv_flag("D2MI_PLAN_NO_LEAN");
+    // odd passes of the register forms walk their chunks backwards
+    // (D2MI_PLAN_NO_REVERSE=1: every pass forwards)
+    const bool reverse_odd = !getenv_flag("D2MI_PLAN_NO_REVERSE");
     const bool forced = force_gm || no_64 || no_reg;  // tuned choices give way to these
     // fixed Euclidean dimensions get straight-line cost code
     const int dim_sel = CM == COST_EUCLID && (d == 2 || d == 3) ? d : 0;
@@ -3148,6 +3156,7 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
           fa.chunk[b] = chunk;
           PlanArgs<T> pa = plan_args(B, chunk, R, upd, resid, keepx, iter == 0);
           pa.resid_part = ctx->part.as<double>() + 3 * part_off;
+          pa.rev = reverse_odd && (iter & 1);
           launch_plan(G, lds, pa, B.kind, lean);
           part_off += G;
         }

@@ -19,12 +19,14 @@
 // centroid supports once per support update), beside the FP32 norms of the
 // unrounded rows: T = |x|^2 + |y|^2 - 2 bf16(x).bf16(y) keeps the rounding
 // errors zero-mean (the distance of the rounded rows would be biased upward
-// by about 2^-16 of |x|^2 + |y|^2, visible in the clustering objective).  The
-// GEMM stages BF16 slices in two LDS buffers (the next slice's global loads
-// are in flight while the matrix cores work on this one, one barrier per
-// slice) and feeds each MFMA operand with one 128-bit LDS read.  With the
-// products 16 times faster than FP32 ones, the 4-byte result table is what
-// bounds this kernel: it is HBM-bound on its output.
+// by about 2^-16 of |x|^2 + |y|^2, visible in the clustering objective).
+// Wide tables (k_pdist2_bf16_dma, 128 x 256 tiles) copy the BF16 slices
+// straight from HBM into three LDS stages, two slices in flight while the
+// matrix cores work on the third; narrow ones (k_pdist2_bf16, 128 x 128)
+// stage them through registers into two LDS buffers.  Each MFMA operand is
+// one 128-bit LDS read.  With the products 16 times faster than FP32 ones,
+// the 4-byte result table is what bounds these kernels: they are HBM-bound
+// on their output.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -72,42 +74,65 @@ __device__ inline void load_norms(const float *nx, const float *ny, int P, int Q
 }
 
 // T = |x|^2 + |y|^2 - 2 acc for one wave's 64 x 64 quarter, stored 32
-// consecutive columns per row.  Entries small next to |x|^2 + |y|^2 are
-// recomputed by direct differences from the FP32 rows, each by the whole
-// wave (coalesced loads over the shared dimension, a shuffle sum), so a
-// wave that holds one pays a few loads rather than a d-step serial loop.
+// consecutive columns per row.  A lane's accumulator registers r = 4 rq + r3
+// (r3 = 0..3) are rows 4 (lane / 32) + 8 rq + r3 of its 32-row block: four
+// consecutive rows, whose norms come in one 16-byte LDS read, tested with
+// one ballot.  Entries small next to |x|^2 + |y|^2 are recomputed by direct
+// differences from the FP32 rows, each by the whole wave (coalesced loads
+// over the shared dimension, a shuffle sum), so a wave that holds one pays a
+// few loads rather than a d-step serial loop.
 __device__ inline void epilogue(const v16f (&acc)[2][2], const float *nxs, const float *nys,
                                 const float *__restrict__ x, const float *__restrict__ y, int P,
                                 int Q, int d, int p0, int q0, int wr, int wc, int lane,
                                 float *__restrict__ T) {
+  const int hi = lane >> 5, lo = lane & 31;
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int c = 0; c < 2; ++c) {
+    const int qc = wc + c * 32 + lo, q = q0 + qc;
+    const float nyq = nys[qc];
 #pragma unroll
-    for (int c = 0; c < 2; ++c)
+    for (int a = 0; a < 2; ++a)
 #pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int i = (r & 3) + 4 * (lane >> 5) + 8 * (r >> 2);
-        const int pr = wr + a * 32 + i, qc = wc + c * 32 + (lane & 31);
-        const int p = p0 + pr, q = q0 + qc;
-        const bool in = p < P && q < Q;
-        const float s = nxs[pr] + nys[qc];
-        float v = s - 2.f * acc[a][c][r];
-        unsigned long long need = __ballot(in && v < RECOMPUTE * s);
-        while (need) {
-          const int l = __ffsll((long long)need) - 1;
-          need &= need - 1;
-          const int pl = __shfl(p, l), ql = __shfl(q, l);
-          const float *xp = x + (size_t)pl * d, *yq = y + (size_t)ql * d;
-          float t = 0.f;
-          for (int k = lane; k < d; k += 64) {
-            const float u = xp[k] - yq[k];
-            t += u * u;
-          }
-          t = wave_sum(t);
-          if (lane == l) v = t;
+      for (int rq = 0; rq < 4; ++rq) {
+        const int pr = wr + a * 32 + 4 * hi + 8 * rq;  // rows pr .. pr + 3
+        const float4 nx4 = *(const float4 *)&nxs[pr];
+        const float nxr[4] = {nx4.x, nx4.y, nx4.z, nx4.w};
+        float v[4];
+        unsigned int flags = 0;  // bit r3: row pr + r3 needs the direct recompute
+#pragma unroll
+        for (int r3 = 0; r3 < 4; ++r3) {
+          const float s = nxr[r3] + nyq;
+          v[r3] = s - 2.f * acc[a][c][4 * rq + r3];
+          if (p0 + pr + r3 < P && q < Q && v[r3] < RECOMPUTE * s) flags |= 1u << r3;
         }
-        if (in) __builtin_nontemporal_store(v < 0.f ? 0.f : v, &T[(size_t)p * Q + q]);
+        if (__ballot(flags != 0)) {
+#pragma unroll
+          for (int r3 = 0; r3 < 4; ++r3) {
+            unsigned long long need = __ballot((flags >> r3) & 1u);
+            while (need) {
+              const int l = __ffsll((long long)need) - 1;
+              need &= need - 1;
+              const int pl = __shfl(p0 + pr + r3, l), ql = __shfl(q, l);
+              const float *xp = x + (size_t)pl * d, *yq = y + (size_t)ql * d;
+              float t = 0.f;
+              for (int k = lane; k < d; k += 64) {
+                const float u = xp[k] - yq[k];
+                t += u * u;
+              }
+              t = wave_sum(t);
+              if (lane == l) v[r3] = t;
+            }
+          }
+        }
+#pragma unroll
+        for (int r3 = 0; r3 < 4; ++r3) {
+          const int p = p0 + pr + r3;
+          if (p < P && q < Q) {
+            __builtin_nontemporal_store(v[r3] < 0.f ? 0.f : v[r3], &T[(size_t)p * Q + q]);
+          }
+        }
       }
+  }
 }
 
 // VEC (d % 4 == 0, so rows start 16-byte aligned): each thread fetches its
@@ -123,7 +148,7 @@ __global__ __launch_bounds__(NTH, 3) void k_pdist2_mfma(const float *__restrict_
                                                      float *__restrict__ T) {
   __shared__ __align__(16) float Xs[TM * LDA];
   __shared__ __align__(16) float Ys[TN * LDA];
-  __shared__ float nxs[TM], nys[TN];
+  __shared__ __align__(16) float nxs[TM], nys[TN];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   // XCD-aware order: consecutive workgroups land on different XCDs, so give
   // each XCD a contiguous band of row tiles (they share the y tiles in L2)
@@ -239,7 +264,7 @@ __global__ __launch_bounds__(128 * WN, WN == 2 ? 3 : 4) void k_pdist2_bf16(
   constexpr int NX = TM * 4 / NT, NY = TQ * 4 / NT;  // 16-byte chunks per thread per slice
   __shared__ __align__(16) __bf16 Xs[2][TM * BLD];
   __shared__ __align__(16) __bf16 Ys[2][TQ * BLD];
-  __shared__ float nxs[TM], nys[TQ];
+  __shared__ __align__(16) float nxs[TM], nys[TQ];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int ntm = (P + TM - 1) / TM, ntn = (Q + TQ - 1) / TQ, nt = ntm * ntn;
   int b;
@@ -316,6 +341,142 @@ __global__ __launch_bounds__(128 * WN, WN == 2 ? 3 : 4) void k_pdist2_bf16(
   epilogue(acc, nxs, nys, x, y, P, Q, d, p0, q0, wr, wc, lane, T);
 }
 
+// The same GEMM with the operand slices copied from HBM straight into LDS
+// (global_load_lds_dwordx4: no staging registers), three LDS stages deep:
+// slices sl + 1 and sl + 2 are in flight while the matrix cores work on
+// slice sl.  A stage holds the tile's 128 x rows and 64 WN y rows, 32 bf16
+// (64 B) each, unpadded; one wave instruction fills 16 rows (1 KB, lane l
+// writing bytes 16 l .. 16 l + 15).  Within a row the four 16-byte chunks
+// are stored rotated, chunk c at position c ^ ((row / 4) % 4), so the 16
+// rows read by one quarter-wave of an MFMA operand load fall on 16
+// different bank groups.  Rows past P or Q load row P - 1 or Q - 1 (their
+// results are never stored).
+constexpr int DMA_STAGES = 3;
+// One wave's 16-byte-per-lane copy from global memory into LDS (lane l's
+// 16 bytes to lds + 16 l), issued by hand: the compiler's own form of this
+// instruction makes every later LDS read wait for all copies in flight
+// (vmcnt(0)), which would drain the pipeline each slice.  One wait state
+// between the M0 write and the copy that reads it.  No register
+// output; the address registers are read at issue (the compiler's form
+// reuses them right after it too).  The callers count these copies in their
+// own s_waitcnt vmcnt and wait for all of them before any other vector
+// memory access is awaited.
+// (M0 is a reserved register: the compiler notes its clobber without
+// tracking it, and nothing else in these kernels uses M0.)
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Winline-asm"
+__device__ __forceinline__ void dma16(const void *gsrc, __bf16 *lds) {
+  const unsigned int m0 =
+      (unsigned int)(size_t)(__attribute__((address_space(3))) __bf16 *)lds;
+  asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off"
+               :
+               : "v"(gsrc), "s"(m0)
+               : "memory", "m0");
+}
+#pragma clang diagnostic pop
+template <int WN>
+__global__ __launch_bounds__(128 * WN, 4) void k_pdist2_bf16_dma(
+    const __bf16 *__restrict__ xb, const __bf16 *__restrict__ yb, int P, int Q, int dp,
+    const float *__restrict__ nx, const float *__restrict__ ny, const float *__restrict__ x,
+    const float *__restrict__ y, int d, float *__restrict__ T) {
+  constexpr int NT = 128 * WN, TQ = 64 * WN, NW = NT / 64;
+  constexpr int XB = TM / 16, YB = TQ / 16, NB = XB + YB;  // 1 KB blocks per stage
+  constexpr int BPW = NB / NW;                             // blocks per wave per stage
+  static_assert(NB % NW == 0, "whole blocks per wave");
+  // one array per stage: the compiler then sees that a stage's reads do not
+  // alias the copies in flight into the other two, and waits for nothing
+  // but its own stage
+  static_assert(DMA_STAGES == 3, "one array per stage below");
+  __shared__ __align__(16) __bf16 St0[(TM + TQ) * BK], St1[(TM + TQ) * BK], St2[(TM + TQ) * BK];
+  __shared__ __align__(16) float nxs[TM], nys[TQ];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int ntm = (P + TM - 1) / TM, ntn = (Q + TQ - 1) / TQ, nt = ntm * ntn;
+  int b;
+  {  // XCD x (= blockIdx % 8) takes a contiguous band of tiles (shared rows in its L2)
+    const int xcd = blockIdx.x % 8, slot = blockIdx.x / 8, q = nt / 8, r = nt % 8;
+    b = xcd * q + min(xcd, r) + slot;
+  }
+  const int tm = b / ntn, tn = b - (b / ntn) * ntn;
+  const int p0 = tm * TM, q0 = tn * TQ;
+  const int wr = (w / WN) * 64, wc = (w % WN) * 64;
+  if (tid < TM) nxs[tid] = p0 + tid < P ? nx[p0 + tid] : 0.f;
+  for (int t = tid; t < TQ; t += NT) nys[t] = q0 + t < Q ? ny[q0 + t] : 0.f;
+  v16f acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[a][c][r] = 0.f;
+  auto stage_of = [&](auto st_c) -> __bf16 * {
+    constexpr int st = decltype(st_c)::value;
+    return st == 0 ? St0 : st == 1 ? St1 : St2;
+  };
+  // this lane's row within a block and the chunk it fetches for its position
+  const int lr = lane >> 2, chunk = (lane & 3) ^ ((lane >> 4) & 3);
+  auto issue = [&](auto st_c, int sl) {
+    __bf16 *St = stage_of(st_c);
+    const int k0 = sl * BK + chunk * 8;
+#pragma unroll
+    for (int i = 0; i < BPW; ++i) {
+      const int blk = w + i * NW;  // wave-uniform
+      const __bf16 *src;
+      if (blk < XB) src = xb + (size_t)min(p0 + blk * 16 + lr, P - 1) * dp + k0;
+      else src = yb + (size_t)min(q0 + (blk - XB) * 16 + lr, Q - 1) * dp + k0;
+      dma16(src, &St[blk * 16 * BK]);
+    }
+  };
+  auto products = [&](auto st_c) {
+    const __bf16 *Xs = stage_of(st_c), *Ys = stage_of(st_c) + TM * BK;
+#pragma unroll
+    for (int ks = 0; ks < BK; ks += 16) {
+      const int kc = (ks >> 3) + (lane >> 5);  // this lane's 16-byte chunk of the slice
+      v8bf av[2], bv[2];
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        const int r = wr + a * 32 + (lane & 31);
+        av[a] = *(const v8bf *)(&Xs[r * BK + ((kc ^ ((r >> 2) & 3)) << 3)]);
+      }
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const int r = wc + c * 32 + (lane & 31);
+        bv[c] = *(const v8bf *)(&Ys[r * BK + ((kc ^ ((r >> 2) & 3)) << 3)]);
+      }
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+          acc[a][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[a], bv[c], acc[a][c], 0, 0, 0);
+    }
+  };
+  using C0 = std::integral_constant<int, 0>;
+  using C1 = std::integral_constant<int, 1>;
+  using C2 = std::integral_constant<int, 2>;
+  const int ns = dp / BK;
+  // slice sl in stage sl % 3
+  auto step = [&](auto cur_c, auto nxt2_c, int sl) {
+    // slice sl has landed once at most the next slice's BPW copies are
+    // outstanding; then a barrier (which also covers the norms' LDS writes
+    // and every wave's reads of the stage refilled below, done at sl - 1)
+    asm volatile("" ::: "memory");
+    if (sl + 1 < ns) __builtin_amdgcn_s_waitcnt(0x0070 | BPW);  // vmcnt(BPW) lgkmcnt(0)
+    else __builtin_amdgcn_s_waitcnt(0x0070);                    // vmcnt(0) lgkmcnt(0)
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+    if (sl + 2 < ns) issue(nxt2_c, sl + 2);
+    products(cur_c);
+  };
+  issue(C0{}, 0);
+  if (ns > 1) issue(C1{}, 1);
+  for (int sl = 0; sl < ns; sl += 3) {
+    step(C0{}, C2{}, sl);
+    if (sl + 1 < ns) step(C1{}, C0{}, sl + 1);
+    if (sl + 2 < ns) step(C2{}, C1{}, sl + 2);
+  }
+  epilogue(acc, nxs, nys, x, y, P, Q, d, p0, q0, wr, wc, lane, T);
+}
+
 __global__ void k_to_f32(const double *__restrict__ a, float *__restrict__ b, size_t n) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x)
@@ -381,11 +542,12 @@ int d2mi_pdist2_bf16_dev(hipStream_t S, int P, int Q, int d, const __bf16 *xb, c
     return D2MI_EUNSUPPORTED;
   }
   // wide tables: 128 x 256 tiles (each x slice staged once serves twice the
-  // columns: 13% faster at 50000 x 1280 x 300); narrow ones: 128 x 128
+  // columns), operands copied into LDS three slices deep; narrow ones:
+  // 128 x 128 with register staging
   if (Q >= 192) {
     const size_t t4 = (size_t)((P + TM - 1) / TM) * ((Q + 255) / 256);
-    k_pdist2_bf16<4><<<(unsigned)t4, 512, 0, S>>>(xb, yb, P, Q, d2mi_pdist2_pad(d), nx, ny, x, y,
-                                                   d, T);
+    k_pdist2_bf16_dma<4><<<(unsigned)t4, 512, 0, S>>>(xb, yb, P, Q, d2mi_pdist2_pad(d), nx, ny, x,
+                                                       y, d, T);
   } else {
     k_pdist2_bf16<2><<<(unsigned)tiles, 256, 0, S>>>(xb, yb, P, Q, d2mi_pdist2_pad(d), nx, ny, x,
                                                       y, d, T);

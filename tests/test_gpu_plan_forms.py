@@ -75,7 +75,8 @@ def test_size_classes_fp64_match_oracle(engine):
     {"D2MI_PLAN_NOREG": "1"},                            # LDS form
     {"D2MI_PLAN_LEGACY": "1"},                           # general form
     {"D2MI_PLAN_LEGACY": "1", "D2MI_BADMM_HBM_SCRATCH": "1"},  # plan scratch in HBM
-], ids=["one_launch", "lds", "general", "hbm_scratch"])
+    {"D2MI_PLAN_NO_REVERSE": "1"},                       # every pass walks its chunk forwards
+], ids=["one_launch", "lds", "general", "hbm_scratch", "forward_only"])
 def test_fp32_forms_agree(engine, env):
     say(f"grid strokes N=4000 fp32: size classes vs {env}")
     data, ph, c0, lab = strokes(4000, seed=1)
