@@ -2315,7 +2315,7 @@ int occupancy_of(F kernel, size_t lds, int threads = NT) {
 // LEAN instances serve passes after the first with no support update,
 // residual or kept X
 enum PlanKind { PK_LEGACY, PK_GM, PK_64, PK_REG8, PK_REG16, PK_TALL, PK_REG8P, PK_REG16P,
-                PK_WAVE8, PK_WAVE16 };
+                PK_WAVE8, PK_WAVE16, PK_REG16W };
 
 const char *plan_kind_name(PlanKind k) {
   switch (k) {
@@ -2328,6 +2328,7 @@ const char *plan_kind_name(PlanKind k) {
     case PK_REG16P: return "reg16p";
     case PK_WAVE8: return "wave8";
     case PK_WAVE16: return "wave16";
+    case PK_REG16W: return "reg16w";
     default: return "general";
   }
 }
@@ -2337,15 +2338,19 @@ inline bool is_tall(PlanKind k) { return k == PK_TALL; }
 // plan copied into LDS ahead of time (k_badmm_plan_reg<..., PF = true>)
 inline bool is_reg(PlanKind k) {
   return k == PK_REG8 || k == PK_REG16 || k == PK_REG8P || k == PK_REG16P || k == PK_WAVE8 ||
-         k == PK_WAVE16;
+         k == PK_WAVE16 || k == PK_REG16W;
 }
 inline bool is_pf(PlanKind k) { return k == PK_REG8P || k == PK_REG16P; }
 // the register form with 64-thread workgroups: one wave per object
 inline bool is_wave(PlanKind k) { return k == PK_WAVE8 || k == PK_WAVE16; }
+// the register form with 512-thread workgroups (FP32): twice the columns of
+// the 256-thread form at the same registers per thread, for objects past
+// its limit (word-embedding documents of 65-128 words against s = 64)
+constexpr int WIDE_NT = 512;
 // workgroup size of a plan kernel: one wave per 64 rows for the tall form,
 // one wave for the wave forms
 inline int plan_threads(PlanKind k, int s) {
-  return is_tall(k) ? (s + 63) / 64 * 64 : is_wave(k) ? 64 : NT;
+  return is_tall(k) ? (s + 63) / 64 * 64 : is_wave(k) ? 64 : k == PK_REG16W ? WIDE_NT : NT;
 }
 
 // d2mi_badmm_tune's request: passes timed per candidate, and the report
@@ -2430,6 +2435,13 @@ struct PlanKernels {
                            : occupancy_of(k_badmm_plan_reg<T, CM, DIM, 16, 1>, lds);
         }
         return 0;
+      case PK_REG16W:
+        if constexpr (F32) {
+          if constexpr (LEAN) return occupancy_of(k_badmm_plan_reg<T, CM, DIM, 16, 0, false, WIDE_NT>, lds, WIDE_NT);
+          return mode == 2 ? occupancy_of(k_badmm_plan_reg<T, CM, DIM, 16, 2, false, WIDE_NT>, lds, WIDE_NT)
+                           : occupancy_of(k_badmm_plan_reg<T, CM, DIM, 16, 1, false, WIDE_NT>, lds, WIDE_NT);
+        }
+        return 0;
       case PK_REG8P:
         if constexpr (PFOK && LEAN) return occupancy_of(k_badmm_plan_reg<T, CM, DIM, 8, 0, true>, lds);
         [[fallthrough]];
@@ -2470,6 +2482,13 @@ struct PlanKernels {
           if constexpr (LEAN) k_badmm_plan_reg<T, CM, DIM, 16, 0><<<G, NT, lds, S>>>(pa);
           else if (mode == 2) k_badmm_plan_reg<T, CM, DIM, 16, 2><<<G, NT, lds, S>>>(pa);
           else k_badmm_plan_reg<T, CM, DIM, 16, 1><<<G, NT, lds, S>>>(pa);
+        }
+        break;
+      case PK_REG16W:
+        if constexpr (F32) {
+          if constexpr (LEAN) k_badmm_plan_reg<T, CM, DIM, 16, 0, false, WIDE_NT><<<G, WIDE_NT, lds, S>>>(pa);
+          else if (mode == 2) k_badmm_plan_reg<T, CM, DIM, 16, 2, false, WIDE_NT><<<G, WIDE_NT, lds, S>>>(pa);
+          else k_badmm_plan_reg<T, CM, DIM, 16, 1, false, WIDE_NT><<<G, WIDE_NT, lds, S>>>(pa);
         }
         break;
       case PK_REG8P:
@@ -2898,6 +2917,13 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
           return s <= 64 && ncols <= (k == PK_REG8 ? 8 : 16) &&
                  Plan64Lds(sizeof(T), sp64, m_lim, s, d, 1, Rmax).total <= ctx->lds_max &&
                  PlanRegLds(sizeof(T), sp64, m_lim, s, d, Rmax).total <= ctx->lds_max;
+        case PK_REG16W: {  // 512 threads: 512 / s columns per slot
+          if (F64 || s > 64) return false;
+          const int wcols = (m_lim + WIDE_NT / s - 1) / (WIDE_NT / s);
+          return wcols <= 16 && m_lim <= WIDE_NT * PRE_W &&
+                 (size_t)m_lim * std::max(d, 1) <= (size_t)WIDE_NT * PRE_X &&
+                 PlanRegLds(sizeof(T), sp64, m_lim, s, d, Rmax, false, WIDE_NT).total <= ctx->lds_max;
+        }
         case PK_WAVE16:
           if (F64) return false;
           [[fallthrough]];
@@ -2938,6 +2964,8 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
           else if (!no_reg && small && fits(B.m_max, PK_WAVE16)) k = PK_WAVE16;
           else if (!no_reg && fits(B.m_max, PK_REG8)) k = PK_REG8;
           else if (!no_reg && fits(B.m_max, PK_REG16)) k = PK_REG16;
+          else if (!no_reg && !getenv_flag("D2MI_PLAN_NO_WIDE") && fits(B.m_max, PK_REG16W))
+            k = PK_REG16W;
           // D2MI_PLAN_PREFETCH=1: the register forms' prefetching variants
           if (getenv_flag("D2MI_PLAN_PREFETCH")) {
             if (k == PK_REG8 && fits(B.m_max, PK_REG8P)) k = PK_REG8P;
@@ -3037,8 +3065,8 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
         // than 3% faster on the plan passes, since more workgroups also cost
         // the per-pass reduction (more records) that these timings leave out
         std::vector<PlanTuneVal> cands{PlanTuneVal{(int)B.kind, 0}};
-        const PlanKind kinds[] = {PK_TALL, PK_WAVE8, PK_WAVE16, PK_REG8, PK_REG16, PK_REG8P,
-                                  PK_REG16P, PK_64, PK_LEGACY, PK_GM};
+        const PlanKind kinds[] = {PK_TALL, PK_WAVE8, PK_WAVE16, PK_REG8, PK_REG16, PK_REG16W,
+                                  PK_REG8P, PK_REG16P, PK_64, PK_LEGACY, PK_GM};
         bool any_fast = false;
         for (PlanKind k : kinds)
           if (k != PK_GM && fits(B.m_max, k)) any_fast = true;
