@@ -926,7 +926,7 @@ struct PlanRegLds {
     rp = o; o = align16(o + (size_t)nt * tsize);
     ids = o; o = align16(o + (size_t)m_max * sizeof(int));
     acc = o; o = align16(o + (size_t)R * sizeof(double));
-    red = o; o = align16(o + 8 * sizeof(double));
+    red = o; o = align16(o + (size_t)(nt / 64 > 8 ? nt / 64 : 8) * sizeof(double));
     up = o; o = align16(o + (R > s ? (size_t)cpi * s * (dd + 1) : 0) * sizeof(double));
     total = o;
   }
@@ -2315,7 +2315,7 @@ int occupancy_of(F kernel, size_t lds, int threads = NT) {
 // LEAN instances serve passes after the first with no support update,
 // residual or kept X
 enum PlanKind { PK_LEGACY, PK_GM, PK_64, PK_REG8, PK_REG16, PK_TALL, PK_REG8P, PK_REG16P,
-                PK_WAVE8, PK_WAVE16, PK_REG16W };
+                PK_WAVE8, PK_WAVE16, PK_REG16W, PK_REG16X };
 
 const char *plan_kind_name(PlanKind k) {
   switch (k) {
@@ -2329,6 +2329,7 @@ const char *plan_kind_name(PlanKind k) {
     case PK_WAVE8: return "wave8";
     case PK_WAVE16: return "wave16";
     case PK_REG16W: return "reg16w";
+    case PK_REG16X: return "reg16x";
     default: return "general";
   }
 }
@@ -2338,19 +2339,24 @@ inline bool is_tall(PlanKind k) { return k == PK_TALL; }
 // plan copied into LDS ahead of time (k_badmm_plan_reg<..., PF = true>)
 inline bool is_reg(PlanKind k) {
   return k == PK_REG8 || k == PK_REG16 || k == PK_REG8P || k == PK_REG16P || k == PK_WAVE8 ||
-         k == PK_WAVE16 || k == PK_REG16W;
+         k == PK_WAVE16 || k == PK_REG16W || k == PK_REG16X;
 }
 inline bool is_pf(PlanKind k) { return k == PK_REG8P || k == PK_REG16P; }
 // the register form with 64-thread workgroups: one wave per object
 inline bool is_wave(PlanKind k) { return k == PK_WAVE8 || k == PK_WAVE16; }
-// the register form with 512-thread workgroups (FP32): twice the columns of
-// the 256-thread form at the same registers per thread, for objects past
-// its limit (word-embedding documents of 65-128 words against s = 64)
-constexpr int WIDE_NT = 512;
+// the register form with 512- and 1024-thread workgroups (FP32): two and
+// four times the columns of the 256-thread form at the same registers per
+// thread, for objects past its limit (word-embedding documents of 65-128 and
+// 129-256 words against s = 64)
+constexpr int WIDE_NT = 512, XWIDE_NT = 1024;
 // workgroup size of a plan kernel: one wave per 64 rows for the tall form,
 // one wave for the wave forms
 inline int plan_threads(PlanKind k, int s) {
-  return is_tall(k) ? (s + 63) / 64 * 64 : is_wave(k) ? 64 : k == PK_REG16W ? WIDE_NT : NT;
+  return is_tall(k)          ? (s + 63) / 64 * 64
+         : is_wave(k)        ? 64
+         : k == PK_REG16W    ? WIDE_NT
+         : k == PK_REG16X    ? XWIDE_NT
+                             : NT;
 }
 
 // d2mi_badmm_tune's request: passes timed per candidate, and the report
@@ -2435,6 +2441,13 @@ struct PlanKernels {
                            : occupancy_of(k_badmm_plan_reg<T, CM, DIM, 16, 1>, lds);
         }
         return 0;
+      case PK_REG16X:
+        if constexpr (F32) {
+          if constexpr (LEAN) return occupancy_of(k_badmm_plan_reg<T, CM, DIM, 16, 0, false, XWIDE_NT>, lds, XWIDE_NT);
+          return mode == 2 ? occupancy_of(k_badmm_plan_reg<T, CM, DIM, 16, 2, false, XWIDE_NT>, lds, XWIDE_NT)
+                           : occupancy_of(k_badmm_plan_reg<T, CM, DIM, 16, 1, false, XWIDE_NT>, lds, XWIDE_NT);
+        }
+        return 0;
       case PK_REG16W:
         if constexpr (F32) {
           if constexpr (LEAN) return occupancy_of(k_badmm_plan_reg<T, CM, DIM, 16, 0, false, WIDE_NT>, lds, WIDE_NT);
@@ -2482,6 +2495,13 @@ struct PlanKernels {
           if constexpr (LEAN) k_badmm_plan_reg<T, CM, DIM, 16, 0><<<G, NT, lds, S>>>(pa);
           else if (mode == 2) k_badmm_plan_reg<T, CM, DIM, 16, 2><<<G, NT, lds, S>>>(pa);
           else k_badmm_plan_reg<T, CM, DIM, 16, 1><<<G, NT, lds, S>>>(pa);
+        }
+        break;
+      case PK_REG16X:
+        if constexpr (F32) {
+          if constexpr (LEAN) k_badmm_plan_reg<T, CM, DIM, 16, 0, false, XWIDE_NT><<<G, XWIDE_NT, lds, S>>>(pa);
+          else if (mode == 2) k_badmm_plan_reg<T, CM, DIM, 16, 2, false, XWIDE_NT><<<G, XWIDE_NT, lds, S>>>(pa);
+          else k_badmm_plan_reg<T, CM, DIM, 16, 1, false, XWIDE_NT><<<G, XWIDE_NT, lds, S>>>(pa);
         }
         break;
       case PK_REG16W:
@@ -2917,12 +2937,14 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
           return s <= 64 && ncols <= (k == PK_REG8 ? 8 : 16) &&
                  Plan64Lds(sizeof(T), sp64, m_lim, s, d, 1, Rmax).total <= ctx->lds_max &&
                  PlanRegLds(sizeof(T), sp64, m_lim, s, d, Rmax).total <= ctx->lds_max;
-        case PK_REG16W: {  // 512 threads: 512 / s columns per slot
+        case PK_REG16W:
+        case PK_REG16X: {  // 512 or 1024 threads: nt / s columns per slot
           if (F64 || s > 64) return false;
-          const int wcols = (m_lim + WIDE_NT / s - 1) / (WIDE_NT / s);
-          return wcols <= 16 && m_lim <= WIDE_NT * PRE_W &&
-                 (size_t)m_lim * std::max(d, 1) <= (size_t)WIDE_NT * PRE_X &&
-                 PlanRegLds(sizeof(T), sp64, m_lim, s, d, Rmax, false, WIDE_NT).total <= ctx->lds_max;
+          const int nt = plan_threads(k, s);
+          const int wcols = (m_lim + nt / s - 1) / (nt / s);
+          return wcols <= 16 && m_lim <= nt * PRE_W &&
+                 (size_t)m_lim * std::max(d, 1) <= (size_t)nt * PRE_X &&
+                 PlanRegLds(sizeof(T), sp64, m_lim, s, d, Rmax, false, nt).total <= ctx->lds_max;
         }
         case PK_WAVE16:
           if (F64) return false;
@@ -2966,6 +2988,8 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
           else if (!no_reg && fits(B.m_max, PK_REG16)) k = PK_REG16;
           else if (!no_reg && !getenv_flag("D2MI_PLAN_NO_WIDE") && fits(B.m_max, PK_REG16W))
             k = PK_REG16W;
+          else if (!no_reg && !getenv_flag("D2MI_PLAN_NO_WIDE") && fits(B.m_max, PK_REG16X))
+            k = PK_REG16X;
           // D2MI_PLAN_PREFETCH=1: the register forms' prefetching variants
           if (getenv_flag("D2MI_PLAN_PREFETCH")) {
             if (k == PK_REG8 && fits(B.m_max, PK_REG8P)) k = PK_REG8P;
@@ -3065,8 +3089,8 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
         // than 3% faster on the plan passes, since more workgroups also cost
         // the per-pass reduction (more records) that these timings leave out
         std::vector<PlanTuneVal> cands{PlanTuneVal{(int)B.kind, 0}};
-        const PlanKind kinds[] = {PK_TALL, PK_WAVE8, PK_WAVE16, PK_REG8, PK_REG16, PK_REG16W,
-                                  PK_REG8P, PK_REG16P, PK_64, PK_LEGACY, PK_GM};
+        const PlanKind kinds[] = {PK_TALL,  PK_WAVE8,  PK_WAVE16, PK_REG8,   PK_REG16, PK_REG16W,
+                                  PK_REG16X, PK_REG8P, PK_REG16P, PK_64, PK_LEGACY, PK_GM};
         bool any_fast = false;
         for (PlanKind k : kinds)
           if (k != PK_GM && fits(B.m_max, k)) any_fast = true;
