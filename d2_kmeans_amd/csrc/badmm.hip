@@ -906,13 +906,14 @@ __device__ __forceinline__ void buf_store(T v, BufRes r, int voff, int soff) {
 }
 
 // LDS layout of k_badmm_plan_reg: only t goes through LDS (for the column
-// sums); column stride sp even, as for k_badmm_plan64.
+// sums); column stride sp even, as for k_badmm_plan64.  Residual passes also
+// keep the Z they started from (z0, same layout as t) for the dual residual.
 struct PlanRegLds {
-  size_t A, col, w, xs, cs, cw, g, rp, ids, acc, red, up, total;
+  size_t A, col, w, xs, cs, cw, g, rp, ids, acc, red, up, z0, total;
   // noA: t lives in the prefetch form's static staging array instead;
-  // nt: threads per workgroup
+  // nt: threads per workgroup; resid: room for z0
   __host__ __device__ PlanRegLds(int tsize, int sp, int m_max, int s, int d, int R,
-                                 bool noA = false, int nt = NT) {
+                                 bool noA = false, int nt = NT, bool resid = false) {
     const size_t dd = d > 0 ? d : 1;
     const int cpi = nt / s;
     size_t o = 0;
@@ -928,6 +929,7 @@ struct PlanRegLds {
     acc = o; o = align16(o + (size_t)R * sizeof(double));
     red = o; o = align16(o + (size_t)(nt / 64 > 8 ? nt / 64 : 8) * sizeof(double));
     up = o; o = align16(o + (R > s ? (size_t)cpi * s * (dd + 1) : 0) * sizeof(double));
+    z0 = o; o = align16(o + (resid ? (size_t)sp * m_max * tsize : 0));
     total = o;
   }
 };
@@ -1002,8 +1004,9 @@ void k_badmm_plan_reg(PlanArgs<T> a) {
   const bool upd = MODE >= 1 && a.upd, first = MODE >= 1 && a.first;
   const bool resid = MODE == 2 && a.resid;
   const bool keepx = MODE >= 1 && (F64 || resid);  // x held until pass 4
-  const PlanRegLds L(sizeof(T), sp, a.m_max, s, d, a.R, PF, NTB);
+  const PlanRegLds L(sizeof(T), sp, a.m_max, s, d, a.R, PF, NTB, resid);
   T *A = PF ? stZ : (T *)(smem + L.A);
+  T *z0l = (T *)(smem + L.z0);  // residual passes: Z before this pass
   T *csl = (T *)(smem + L.col);
   T *wl = (T *)(smem + L.w), *xs = (T *)(smem + L.xs);
   T *cs = (T *)(smem + L.cs), *cwl = (T *)(smem + L.cw), *g = (T *)(smem + L.g);
@@ -1172,6 +1175,7 @@ void k_badmm_plan_reg(PlanArgs<T> a) {
       if (j < m) {
         const T cost = CM == COST_TABLE ? cr[c] : cost_k(j);
         const T z = reinit ? wl[j] * cwk : zr[c];
+        if (resid) z0l[ke + j * sp] = z;  // read back by this thread in pass 4
         const T t = z * plan_exp<T>(-(scale_cost<T>(cost, rho, inv_rho) + yr[c])) + (T)1e-9;
         A[ke + j * sp] = t;
         zr[c] = t;
@@ -1261,7 +1265,7 @@ void k_badmm_plan_reg(PlanArgs<T> a) {
           for (int u = 0; u < U && c0 + u < NC; ++u) {
             const int j = jb + (c0 + u) * cpi;
             if (j < m) {
-              z0v[u] = reinit ? wl[j] * cwk : buf_load<T>(rz, voff, (c0 + u) * cstride);
+              z0v[u] = z0l[ke + j * sp];
               cv[u] = cost_k(j);
             }
           }
@@ -2936,7 +2940,7 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
         case PK_REG8:
           return s <= 64 && ncols <= (k == PK_REG8 ? 8 : 16) &&
                  Plan64Lds(sizeof(T), sp64, m_lim, s, d, 1, Rmax).total <= ctx->lds_max &&
-                 PlanRegLds(sizeof(T), sp64, m_lim, s, d, Rmax).total <= ctx->lds_max;
+                 PlanRegLds(sizeof(T), sp64, m_lim, s, d, Rmax, false, NT, true).total <= ctx->lds_max;
         case PK_REG16W:
         case PK_REG16X: {  // 512 or 1024 threads: nt / s columns per slot
           if (F64 || s > 64) return false;
@@ -2944,7 +2948,7 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
           const int wcols = (m_lim + nt / s - 1) / (nt / s);
           return wcols <= 16 && m_lim <= nt * PRE_W &&
                  (size_t)m_lim * std::max(d, 1) <= (size_t)nt * PRE_X &&
-                 PlanRegLds(sizeof(T), sp64, m_lim, s, d, Rmax, false, nt).total <= ctx->lds_max;
+                 PlanRegLds(sizeof(T), sp64, m_lim, s, d, Rmax, false, nt, true).total <= ctx->lds_max;
         }
         case PK_WAVE16:
           if (F64) return false;
@@ -2954,7 +2958,7 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
           const int wcols = (m_lim + 64 / s - 1) / (64 / s);
           return wcols <= (k == PK_WAVE8 ? 8 : 16) && m_lim <= 64 * PRE_W &&
                  (size_t)m_lim * std::max(d, 1) <= (size_t)64 * PRE_X &&
-                 PlanRegLds(sizeof(T), sp64, m_lim, s, d, Rmax, false, 64).total <= ctx->lds_max;
+                 PlanRegLds(sizeof(T), sp64, m_lim, s, d, Rmax, false, 64, true).total <= ctx->lds_max;
         }
         case PK_64:
           return s <= 64 && Plan64Lds(sizeof(T), sp64, m_lim, s, d, 1, Rmax).total <= ctx->lds_max;
@@ -3011,11 +3015,13 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
     }
     // dynamic LDS of bucket B's kernel for a pass; lean: the prefetch form's
     // lean passes keep t in their static staging array
-    auto lds_of = [&](const Bucket &B, int keepx, int R, bool lean = false) -> size_t {
+    auto lds_of = [&](const Bucket &B, int keepx, int R, bool lean = false,
+                      bool resid = false) -> size_t {
       if (is_tall(B.kind)) return 0;  // static LDS only
       if (is_reg(B.kind))
         return PlanRegLds(sizeof(T), B.sp, B.m_max, s, d, R, lean && is_pf(B.kind),
-                          plan_threads(B.kind, s)).total;
+                          plan_threads(B.kind, s), resid)
+            .total;
       return B.k64 ? Plan64Lds(sizeof(T), B.sp, B.m_max, s, d, keepx, R).total
                    : PlanLds(sizeof(T), B.sp, B.m_max, s, d, keepx, R, B.gm).total;
     };
@@ -3153,7 +3159,7 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
     int part_total = 0;
     for (int b = 0; b < nbk; ++b) {
       Bucket &B = bk[b];
-      const size_t lds_full = lds_of(B, 1, Rmax);
+      const size_t lds_full = lds_of(B, 1, Rmax, false, true);
       if (lds_full > ctx->lds_max) {
         d2mi_set_error("B-ADMM with s = %d, d = %d needs %zu B of LDS (limit %zu)", s, d,
                        lds_full, ctx->lds_max);
@@ -3198,7 +3204,7 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
           fa.chunk[b] = 1;
           if (B.count == 0) continue;
           const bool lean = !F64 && !B.gm && !no_lean && iter > 0 && !upd && !resid && !keepx;
-          const size_t lds = lds_of(B, keepx, R, lean);
+          const size_t lds = lds_of(B, keepx, R, lean, resid);
           int G, chunk;
           geometry(B, lds, &G, &chunk, lean, resid);
           if (G > B.Gmax) {  // records are sized for Gmax workgroups
