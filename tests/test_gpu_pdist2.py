@@ -71,3 +71,26 @@ def test_pdist2_mfma_bf16(engine, n, m, d):
     if near.any():
         assert np.max(np.abs(got - ref)[near] / np.maximum(ref[near], 1e-7)) < 1e-5
     assert (got >= 0).all()
+
+
+@pytest.mark.parametrize("d", [13, 40, 96, 300])
+def test_pdist2_bf16_wide_equals_narrow(engine, d):
+    """Tables of >= 192 columns take the kernel that copies its BF16 slices
+    straight into three LDS stages (rotated 16-byte chunks); narrower ones
+    the register-staged kernel.  Both multiply the same BF16 operands in the
+    same order of the shared dimension, so a 256-column table must equal, bit
+    for bit, the two 128-column tables of its halves; d covers 1, 2, 3 and 10
+    slices of the three-stage pipeline."""
+    say(f"pdist2 bf16 wide vs narrow, d={d}")
+    rng = np.random.default_rng(11 + d)
+    m, n = 777, 256
+    A = rng.standard_normal((n, d))
+    B = rng.standard_normal((m, d))
+    A /= np.linalg.norm(A, axis=1, keepdims=True)
+    B /= np.linalg.norm(B, axis=1, keepdims=True)
+    B[:40] = A[:40] + 1e-4 * rng.standard_normal((40, d))  # recomputed entries too
+    wide = engine.pdist2_mfma(A, B, bf16=True).reshape(m, n)
+    lo = engine.pdist2_mfma(A[:128], B, bf16=True).reshape(m, 128)
+    hi = engine.pdist2_mfma(A[128:], B, bf16=True).reshape(m, 128)
+    np.testing.assert_array_equal(wide[:, :128], lo)
+    np.testing.assert_array_equal(wide[:, 128:], hi)
