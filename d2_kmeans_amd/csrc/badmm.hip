@@ -1003,7 +1003,11 @@ void k_badmm_plan_reg(PlanArgs<T> a) {
   const int lane = tid & 63, wv = tid >> 6;
   const bool upd = MODE >= 1 && a.upd, first = MODE >= 1 && a.first;
   const bool resid = MODE == 2 && a.resid;
-  const bool keepx = MODE >= 1 && (F64 || resid);  // x held until pass 4
+  // x held until pass 4 in FP64 only (its Y update keeps the reference's
+  // Y + (X - Z) order); FP32 residual passes take x for the residuals from
+  // t (still in LDS from pass 1) times the column scale (from pass 2): the
+  // product pass 3 formed, so the same value, without 16 more registers
+  const bool keepx = MODE >= 1 && F64;
   const PlanRegLds L(sizeof(T), sp, a.m_max, s, d, a.R, PF, NTB, resid);
   T *A = PF ? stZ : (T *)(smem + L.A);
   T *z0l = (T *)(smem + L.z0);  // residual passes: Z before this pass
@@ -1277,8 +1281,9 @@ void k_badmm_plan_reg(PlanArgs<T> a) {
             const T z = F64 ? (zr[c] / gk) * cwk : zr[c] * gk;
             if (resid) {
               const T cost = scale_cost<T>(cv[u], rho, inv_rho);
-              r_obj += (double)cost * (double)xr[c];
-              r_pr += fabs((double)xr[c] - (double)z);
+              const T xv = keepx ? xr[c] : A[ke + j * sp] * csl[j];
+              r_obj += (double)cost * (double)xv;
+              r_pr += fabs((double)xv - (double)z);
               r_dr += fabs((double)z0v[u] - (double)z);
             }
             yr[c] = keepx ? yr[c] + (xr[c] - z) : yr[c] - z;
@@ -1299,7 +1304,7 @@ void k_badmm_plan_reg(PlanArgs<T> a) {
         }
       }
     }
-    if (resid) __syncthreads();  // pass 4 read wl / xs: keep them until all waves are done
+    if (resid) __syncthreads();  // pass 4 read wl / xs / t / csl: keep them until all waves are done
   }
   __syncthreads();
   if (cur >= 0)
