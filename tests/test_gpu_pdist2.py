@@ -94,3 +94,29 @@ def test_pdist2_bf16_wide_equals_narrow(engine, d):
     hi = engine.pdist2_mfma(A[128:], B, bf16=True).reshape(m, 128)
     np.testing.assert_array_equal(wide[:, :128], lo)
     np.testing.assert_array_equal(wide[:, 128:], hi)
+
+
+@pytest.mark.parametrize("bf16", [False, True])
+def test_pdist2_concentrated_recomputes(engine, bf16):
+    """Entries that lose digits to cancellation are recomputed by direct
+    differences of the FP32 rows.  Many centroid rows equal (or within 1e-4
+    of) the same few vocabulary rows, as popular words make them, which puts
+    hundreds of recomputes in a few tiles; they must equal the FP64
+    distances of the FP32 rows to FP32 rounding, and the rest of the table
+    must be untouched by them."""
+    say(f"pdist2 concentrated recomputes, bf16={bf16}")
+    rng = np.random.default_rng(29)
+    V, Q, d = 3000, 320, 300
+    vocab = rng.standard_normal((V, d))
+    vocab /= np.linalg.norm(vocab, axis=1, keepdims=True)
+    cs = vocab[rng.integers(0, 8, Q)] + np.where(rng.random((Q, 1)) < 0.5, 0.0, 1e-4) * \
+        rng.standard_normal((Q, d))
+    T = engine.pdist2_mfma(cs, vocab, bf16=bf16).reshape(V, Q)
+    ref = O.pdist2(cs.astype(np.float32).astype(np.float64),
+                   vocab.astype(np.float32).astype(np.float64)).reshape(V, Q)
+    small = ref < 1e-3
+    assert small.sum() >= Q // 2
+    assert np.max(np.abs(T[small] - ref[small])) < 1e-6
+    # the rest: FP32 rounding of |x|^2 + |y|^2 - 2 x.y (|x|^2 + |y|^2 = 2), or the
+    # BF16 products' rounding (test_pdist2_mfma_bf16 pins that error exactly)
+    assert np.max(np.abs(T[~small] - ref[~small])) < (1e-2 if bf16 else 1e-5)
