@@ -907,7 +907,9 @@ __device__ __forceinline__ void buf_store(T v, BufRes r, int voff, int soff) {
 
 // LDS layout of k_badmm_plan_reg: only t goes through LDS (for the column
 // sums); column stride sp even, as for k_badmm_plan64.  Residual passes also
-// keep the Z they started from (z0, same layout as t) for the dual residual.
+// keep the Z they started from (z0, same layout as t) for the dual residual,
+// and two copies of the object's supports and bin ids (xs, ids), used by
+// alternate objects.
 struct PlanRegLds {
   size_t A, col, w, xs, cs, cw, g, rp, ids, acc, red, up, z0, total;
   // noA: t lives in the prefetch form's static staging array instead;
@@ -920,12 +922,12 @@ struct PlanRegLds {
     A = o; o = align16(o + (noA ? 0 : (size_t)sp * m_max * tsize));
     col = o; o = align16(o + (size_t)m_max * tsize);
     w = o; o = align16(o + (size_t)m_max * tsize);
-    xs = o; o = align16(o + (size_t)m_max * dd * tsize);
+    xs = o; o = align16(o + (size_t)(resid ? 2 : 1) * m_max * dd * tsize);
     cs = o; o = align16(o + (size_t)s * dd * tsize);
     cw = o; o = align16(o + (size_t)s * tsize);
     g = o; o = align16(o + (size_t)s * tsize);
     rp = o; o = align16(o + (size_t)nt * tsize);
-    ids = o; o = align16(o + (size_t)m_max * sizeof(int));
+    ids = o; o = align16(o + (size_t)(resid ? 2 : 1) * m_max * sizeof(int));
     acc = o; o = align16(o + (size_t)R * sizeof(double));
     red = o; o = align16(o + (size_t)(nt / 64 > 8 ? nt / 64 : 8) * sizeof(double));
     up = o; o = align16(o + (R > s ? (size_t)cpi * s * (dd + 1) : 0) * sizeof(double));
@@ -1058,6 +1060,11 @@ void k_badmm_plan_reg(PlanArgs<T> a) {
     // the compiler keeps every column slot's addresses live across objects
     int ke = ke_t, jb = jb_t;
     asm volatile("" : "+v"(ke), "+v"(jb));
+    // residual passes: this object's supports and bin ids go to the copy the
+    // previous object did not use, whose pass 4 may still read its own
+    const int bo = resid ? (i & 1) : 0;
+    T *xo = xs + (size_t)bo * a.m_max * (d > 0 ? d : 1);
+    int *io = ids + bo * a.m_max;
     const int o = const_load(a.perm + p);
     const int l = const_load(a.label + o);
     const int m = const_load(a.p_str + o);
@@ -1129,14 +1136,14 @@ void k_badmm_plan_reg(PlanArgs<T> a) {
       const int q = tid + r * NTB;
       if (q < m) {
         wl[q] = wpre[r];
-        if constexpr (CM == COST_SPARSE) ids[q] = ipre[r];
+        if constexpr (CM == COST_SPARSE) io[q] = ipre[r];
       }
     }
     if constexpr (CM == COST_EUCLID) {
 #pragma unroll
       for (int r = 0; r < PRE_X; ++r) {
         const int q = tid + r * NTB;
-        if (q < m * d) xs[q] = xpre[r];
+        if (q < m * d) xo[q] = xpre[r];
       }
     }
     __syncthreads();  // also: every wave has finished the previous object's last pass
@@ -1153,12 +1160,12 @@ void k_badmm_plan_reg(PlanArgs<T> a) {
         if constexpr (DIM > 0) {
 #pragma unroll
           for (int dd = 0; dd < DIM; ++dd) {
-            const T u = csk[dd] - xs[j * DIM + dd];
+            const T u = csk[dd] - xo[j * DIM + dd];
             c += u * u;
           }
         } else {
           for (int dd = 0; dd < d; ++dd) {
-            const T u = cs[ke * d + dd] - xs[j * d + dd];
+            const T u = cs[ke * d + dd] - xo[j * d + dd];
             c += u * u;
           }
         }
@@ -1166,7 +1173,7 @@ void k_badmm_plan_reg(PlanArgs<T> a) {
       } else if constexpr (CM == COST_HIST) {
         return a.dist[ke + (size_t)j * s];
       } else if constexpr (CM == COST_SPARSE) {
-        return a.dist[(size_t)ids[j] * a.V + ke];
+        return a.dist[(size_t)io[j] * a.V + ke];
       } else {
         return a.ctab[base + ke + j * s];
       }
@@ -1222,7 +1229,7 @@ void k_badmm_plan_reg(PlanArgs<T> a) {
           if (upd) {
             rx += (double)x;  // row sums of X (:211)
             if constexpr (CM == COST_EUCLID)
-              for (int dd = 0; dd < d; ++dd) P[dd] += (double)x * (double)xs[j * d + dd];  // (:206)
+              for (int dd = 0; dd < d; ++dd) P[dd] += (double)x * (double)xo[j * d + dd];  // (:206)
             if constexpr (CM == COST_TABLE) a.xout[base + ke + j * s] = x;  // for the support GEMM
           }
         }
@@ -1258,9 +1265,16 @@ void k_badmm_plan_reg(PlanArgs<T> a) {
     }
     if constexpr (PF) lds_sync();
     else __syncthreads();
-    // pass 4 -- step 2c + 3: z = z' / rowsum * c_w (:169-170); Y += X - Z (:175)
+    // pass 4 -- step 2c + 3: z = z' / rowsum * c_w (:169-170); Y += X - Z (:175).
+    // Residual passes read, besides registers, LDS the next object leaves
+    // alone until this pass is done everywhere: this thread's own t and Z0
+    // slots, csl and g (rewritten after two more barriers), this object's copy
+    // of the supports and bin ids (the next object fills the other copy) --
+    // so no barrier follows this pass.  An object's terms are summed in T and
+    // added to the FP64 totals once.
     {
       const T gk = g[ke];
+      T ob = 0, pr = 0, dr = 0;
 #pragma unroll
       for (int c0 = 0; c0 < NC; c0 += U) {
         T z0v[U], cv[U];
@@ -1282,9 +1296,9 @@ void k_badmm_plan_reg(PlanArgs<T> a) {
             if (resid) {
               const T cost = scale_cost<T>(cv[u], rho, inv_rho);
               const T xv = keepx ? xr[c] : A[ke + j * sp] * csl[j];
-              r_obj += (double)cost * (double)xv;
-              r_pr += fabs((double)xv - (double)z);
-              r_dr += fabs((double)z0v[u] - (double)z);
+              ob += cost * xv;
+              pr += fabs(xv - z);
+              dr += fabs(z0v[u] - z);
             }
             yr[c] = keepx ? yr[c] + (xr[c] - z) : yr[c] - z;
             zr[c] = z;
@@ -1294,6 +1308,11 @@ void k_badmm_plan_reg(PlanArgs<T> a) {
       // the stores, from registers that stay untouched while they drain:
       // every value is materialised first (an instruction that rewrites a
       // register an in-flight store reads must wait for that store)
+      if (resid) {
+        r_obj += (double)ob;
+        r_pr += (double)pr;
+        r_dr += (double)dr;
+      }
 #pragma unroll
       for (int c = 0; c < NC; ++c) asm volatile("" : "+v"(zr[c]), "+v"(yr[c]));
 #pragma unroll
@@ -1304,7 +1323,6 @@ void k_badmm_plan_reg(PlanArgs<T> a) {
         }
       }
     }
-    if (resid) __syncthreads();  // pass 4 read wl / xs / t / csl: keep them until all waves are done
   }
   __syncthreads();
   if (cur >= 0)
