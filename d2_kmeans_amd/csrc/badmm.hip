@@ -1916,20 +1916,298 @@ __global__ __launch_bounds__(NT) void k_badmm_supp_mfma(SuppArgs<float> a) {
   }
 }
 
+// ---- step 5 numerators over packed columns, copied straight into LDS
+//
+// The same numerators as k_badmm_supp_mfma, with the work laid out once per
+// update: each label's columns (its objects in label-sorted order, each
+// object's columns in order) are packed into blocks of SD_JB, the label's
+// last block padded, the labels' blocks back to back.  For block b,
+// desc[32 b + c] is column c as a plan column q (X at s q; -1: padding, read
+// from a zero buffer) and desc[32 b + 16 + c] the row of its weight vector
+// (the word id, or q for supports; 0 for padding); blab[b] is its label,
+// bs[l] label l's first block and bs[K] the number of blocks.
+constexpr int SD_JB = 16, SD_NT = 256, SD_DSLOTS = 5;
+constexpr int SD_TPW_MAX = 5;  // 32 x 32 tiles per wave: 320 output columns per sweep
+
+// cum[i] = sum of the first i objects' column counts in the order perm: each
+// block of NT objects scanned (k_cumw_local, block totals to bt), then the
+// totals of the blocks before added (k_cumw_carry)
+__global__ __launch_bounds__(NT) void k_cumw_local(const int *__restrict__ perm,
+                                                   const int *__restrict__ p_str, int n,
+                                                   long long *__restrict__ cum,
+                                                   long long *__restrict__ bt) {
+  __shared__ long long buf[NT];
+  const int tid = threadIdx.x, i = blockIdx.x * NT + tid;
+  buf[tid] = i < n ? (long long)p_str[perm[i]] : 0;
+  __syncthreads();
+  for (int off = 1; off < NT; off <<= 1) {  // inclusive Hillis-Steele scan
+    const long long v = tid >= off ? buf[tid - off] : 0;
+    __syncthreads();
+    buf[tid] += v;
+    __syncthreads();
+  }
+  if (i < n) cum[i + 1] = buf[tid];
+  if (i == 0) cum[0] = 0;
+  if (tid == NT - 1) bt[blockIdx.x] = buf[NT - 1];
+}
+__global__ __launch_bounds__(NT) void k_cumw_carry(int n, const long long *__restrict__ bt,
+                                                   long long *__restrict__ cum) {
+  __shared__ long long red[NT];
+  const int tid = threadIdx.x, i = blockIdx.x * NT + tid;
+  long long t = 0;
+  for (int b = tid; b < (int)blockIdx.x; b += NT) t += bt[b];
+  red[tid] = t;
+  __syncthreads();
+  for (int h = NT / 2; h > 0; h >>= 1) {
+    if (tid < h) red[tid] += red[tid + h];
+    __syncthreads();
+  }
+  if (i < n) cum[i + 1] += red[0];
+}
+// bs[l]: first block of label l (labels' column counts from cum at the label
+// segments seg of perm, rounded up to whole blocks)
+__global__ void k_supp_blocks(int K, const int *__restrict__ seg, const long long *__restrict__ cum,
+                              int *__restrict__ bs) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    long long run = 0;
+    for (int l = 0; l < K; ++l) {
+      bs[l] = (int)run;
+      run += (cum[seg[l + 1]] - cum[seg[l]] + SD_JB - 1) / SD_JB;
+    }
+    bs[K] = (int)run;
+  }
+}
+// one wave per object (perm position p): its columns' descriptors
+template <bool WORD>
+__global__ __launch_bounds__(64) void k_supp_desc(const int *__restrict__ perm,
+                                                  const int *__restrict__ label,
+                                                  const int *__restrict__ p_str,
+                                                  const long long *__restrict__ ocum,
+                                                  const int *__restrict__ sym,
+                                                  const int *__restrict__ seg,
+                                                  const long long *__restrict__ cum,
+                                                  const int *__restrict__ bs,
+                                                  int *__restrict__ desc) {
+  const int p = blockIdx.x, o = perm[p], l = label[o], m = p_str[o];
+  const long long q0 = ocum[o];
+  const long long pos = (long long)SD_JB * bs[l] + (cum[p] - cum[seg[l]]);
+  for (int j = threadIdx.x; j < m; j += 64) {
+    const long long x = pos + j, b = x / SD_JB;
+    const int c = (int)(x - b * SD_JB), q = (int)(q0 + j);
+    desc[32 * b + c] = q;
+    desc[32 * b + 16 + c] = WORD ? sym[q] : q;
+  }
+}
+// one workgroup per label: its padding columns and its blocks' label
+__global__ __launch_bounds__(64) void k_supp_pad(const int *__restrict__ seg,
+                                                 const long long *__restrict__ cum,
+                                                 const int *__restrict__ bs,
+                                                 int *__restrict__ desc, int *__restrict__ blab) {
+  const int l = blockIdx.x;
+  const long long cols = cum[seg[l + 1]] - cum[seg[l]];
+  const long long x0 = (long long)SD_JB * bs[l] + cols, x1 = (long long)SD_JB * bs[l + 1];
+  for (long long x = x0 + threadIdx.x; x < x1; x += 64) {
+    const long long b = x / SD_JB;
+    const int c = (int)(x - b * SD_JB);
+    desc[32 * b + c] = -1;
+    desc[32 * b + 16 + c] = 0;
+  }
+  for (int b = bs[l] + threadIdx.x; b < bs[l + 1]; b += 64) blab[b] = l;
+}
+
+struct SuppDmaArgs {
+  int s, d, G;
+  const int *__restrict__ desc;
+  const int *__restrict__ blab;
+  const int *__restrict__ nblk;  // bs[K]
+  const float *__restrict__ X;   // plan layout, from the support-update pass
+  const float *__restrict__ W;   // word vectors or supports, rows of d
+  const float *__restrict__ zeros;  // 64 zero floats
+  double *__restrict__ rec;      // record (g, l) at g + l, s x d each
+};
+
+// One wave's copy from global memory into LDS, issued by hand (the
+// compiler's own form makes every later LDS read wait for all copies in
+// flight): 4 or 16 bytes per lane, lane l's bytes to lds + 4 l or 16 l, lds
+// the same for every lane.  One wait state between the M0 write and the
+// copy.  No register output; the callers count these copies in their own
+// s_waitcnt vmcnt.
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Winline-asm"
+__device__ __forceinline__ void lds_copy16(const void *gsrc, void *lds) {
+  const unsigned int m0 = __builtin_amdgcn_readfirstlane(
+      (unsigned int)(size_t)(__attribute__((address_space(3))) char *)lds);  // wave-uniform
+  asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off"
+               :
+               : "v"(gsrc), "s"(m0)
+               : "memory", "m0");
+}
+__device__ __forceinline__ void lds_copy4(const void *gsrc, void *lds) {
+  const unsigned int m0 = __builtin_amdgcn_readfirstlane(
+      (unsigned int)(size_t)(__attribute__((address_space(3))) char *)lds);  // wave-uniform
+  asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dword %0, off"
+               :
+               : "v"(gsrc), "s"(m0)
+               : "memory", "m0");
+}
+#pragma clang diagnostic pop
+
+// Workgroup g takes an equal run of blocks (so equal work).  Per sweep of
+// 64 TPW output columns, block i goes through three LDS stages: its
+// descriptors are copied into one of five slots four blocks ahead, its X
+// columns and weight rows two blocks ahead (X: wave w copies columns 4 w ..
+// 4 w + 3, 16 bytes per lane; W: wave w copies pieces w + 4 u of the
+// block's flattened 16 x 64 TPW rows), so blocks i + 1 and i + 2 are in
+// flight while the matrix cores multiply block i.  Wave w owns 32 x 32
+// output tiles f = w TPW + u (rows 32 (f % 2), columns 32 (f / 2)), FP32
+// MFMA accumulators; a label change flushes them into the FP64 record
+// (g, l).  Needs s % 4 == 0, s <= 64 and d % 4 == 0: rows past s and
+// columns past d are read from row 0 or column 0 (finite) and never stored.
+template <int TPW>
+__global__ __launch_bounds__(SD_NT, 2) void k_badmm_supp_dma(SuppDmaArgs a) {
+  constexpr int TD = 64 * TPW, XST = SD_JB * 64, STG = XST + SD_JB * TD;
+  __shared__ __align__(16) float St[3 * STG];  // stage i % 3 at St + (i % 3) STG
+  __shared__ __align__(16) int Dsc[SD_DSLOTS][32];
+  typedef float v16f __attribute__((ext_vector_type(16)));
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int s = a.s, d = a.d, g = blockIdx.x;
+  const size_t R = (size_t)s * d;
+  const int nblk = const_load(a.nblk);
+  const int nbw = (nblk + a.G - 1) / a.G;
+  const int b0 = min(nblk, g * nbw), nb = min(nblk, b0 + nbw) - b0;
+  if (nb <= 0) return;
+  // the helpers below are forced inline: a call would keep acc in scratch
+  v16f acc[TPW];
+  auto zero = [&]() __attribute__((always_inline)) {
+#pragma unroll
+    for (int u = 0; u < TPW; ++u)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[u][r] = 0.f;
+  };
+  // record (g, l) as a buffer: the lane's offset in one VGPR, made opaque so
+  // the 16 TPW store addresses are not hoisted out of the block loop
+  auto flush = [&](int l, int t0) __attribute__((always_inline)) {
+    const BufRes rr = buf_res(a.rec + (size_t)(g + l) * R);
+    int ln = lane;
+    asm volatile("" : "+v"(ln));
+    const int kl = 4 * (ln >> 5), tl = t0 + (ln & 31);
+    const int voff = (kl * d + tl) * (int)sizeof(double);
+#pragma unroll
+    for (int u = 0; u < TPW; ++u) {
+      const int f = w * TPW + u, tb = (f >> 1) * 32;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int kr = (f & 1) * 32 + (r & 3) + 8 * (r >> 2);  // row less the lane part
+        if (kr + kl < s && tl + tb < d)
+          buf_store<double>((double)acc[u][r], rr, voff, (kr * d + tb) * (int)sizeof(double));
+      }
+    }
+  };
+  auto stage_of = [&](int i) __attribute__((always_inline)) -> float * { return St + (i % 3) * STG; };
+  auto issue_desc = [&](int j) __attribute__((always_inline)) {  // block b0 + j: wave w copies descriptor ints 8 w .. 8 w + 7
+    if (lane < 8) lds_copy4(a.desc + (size_t)(b0 + j) * 32 + 8 * w + lane, &Dsc[j % SD_DSLOTS][8 * w]);
+  };
+  auto issue_data = [&](int j, int t0) __attribute__((always_inline)) {
+    float *S = stage_of(j);
+    const int *D = Dsc[j % SD_DSLOTS];
+    {
+      const int c = 4 * w + (lane >> 4), r4 = 4 * (lane & 15);
+      const int q = D[c];
+      const float *src = q >= 0 ? a.X + (size_t)s * q + (r4 < s ? r4 : 0) : a.zeros + r4;
+      lds_copy16(src, S + 4 * w * 64);
+    }
+#pragma unroll
+    for (int u = 0; u < TPW; ++u) {
+      const int f = (w + 4 * u) * 64 + lane;  // 16-byte piece of the flattened block
+      const int r = f / (TD / 4), t = t0 + 4 * (f - r * (TD / 4));
+      const float *src = a.W + (size_t)D[16 + r] * d + (t < d ? t : 0);
+      lds_copy16(src, S + XST + (w + 4 * u) * 256);
+    }
+  };
+  auto products = [&](int i) __attribute__((always_inline)) {
+    const float *Xs = stage_of(i), *Ws = Xs + XST;
+#pragma unroll
+    for (int kk = 0; kk < SD_JB; kk += 2) {
+      const int kq = kk + (lane >> 5);
+      const float a0 = Xs[kq * 64 + (lane & 31)], a1 = Xs[kq * 64 + 32 + (lane & 31)];
+#pragma unroll
+      for (int u = 0; u < TPW; ++u) {
+        const int f = w * TPW + u;
+        const float bv = Ws[kq * TD + (f >> 1) * 32 + (lane & 31)];
+        acc[u] = __builtin_amdgcn_mfma_f32_32x32x2f32((f & 1) ? a1 : a0, bv, acc[u], 0, 0, 0);
+      }
+    }
+  };
+  // block i's copies have landed once at most the copies issued after them
+  // are outstanding: block i + 1's data and block i + 3's descriptors (one
+  // counter orders them; stores issued in between only make the wait
+  // stricter); then a barrier, after which every wave is also done with
+  // block i - 1's stage and descriptor slot, refilled below
+  constexpr int PER_BLOCK = 1 + TPW;  // data copies per wave per block
+  auto wait_block = [&](int i) __attribute__((always_inline)) {
+    asm volatile("" ::: "memory");
+    if (i + 3 < nb) __builtin_amdgcn_s_waitcnt(0x0070 | (PER_BLOCK + 1));  // vmcnt(n) lgkmcnt(0)
+    else if (i + 1 < nb) __builtin_amdgcn_s_waitcnt(0x0070 | PER_BLOCK);
+    else __builtin_amdgcn_s_waitcnt(0x0070);
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+  };
+  static_assert(PER_BLOCK + 1 <= 15, "vmcnt in the low field");
+  for (int t0 = 0; t0 < d; t0 += TD) {
+    zero();
+    int cur = -1;
+    issue_desc(0);
+    if (nb > 1) issue_desc(1);
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_waitcnt(0x0070);
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+    if (nb > 2) issue_desc(2);
+    issue_data(0, t0);
+    if (nb > 3) issue_desc(3);
+    if (nb > 1) issue_data(1, t0);
+    for (int i = 0; i < nb; ++i) {
+      wait_block(i);
+      if (i + 4 < nb) issue_desc(i + 4);
+      if (i + 2 < nb) issue_data(i + 2, t0);
+      const int l = const_load(a.blab + b0 + i);
+      if (l != cur) {
+        if (cur >= 0) flush(cur, t0);
+        zero();
+        cur = l;
+      }
+      products(i);
+    }
+    flush(cur, t0);
+    __syncthreads();  // stages and descriptor slots are reused by the next sweep
+  }
+}
+
 // new supports = numerators / X row sums (_dirms, :219-221), records summed
 // in workgroup order
 template <typename T>
 __global__ __launch_bounds__(NT) void k_badmm_supp_fin(int s, int d, int chunk,
                                                        const int *__restrict__ seg,
+                                                       const int *__restrict__ bs, int G,
                                                        const double *__restrict__ rec,
                                                        const double *__restrict__ rx,
                                                        double *__restrict__ cs64,
                                                        T *__restrict__ csT, int *flags) {
   // grid (ceil(s d / NT), K): one thread per (label, support coordinate); each
-  // sums its label's workgroup records in workgroup order
+  // sums its label's workgroup records in workgroup order.  Records of
+  // k_badmm_supp_dma (bs: its label block starts; G workgroups of equal runs
+  // of blocks), else of chunks of chunk objects over the label segments seg.
   const int l = blockIdx.y;
-  const int lo = seg[l], hi = seg[l + 1];
-  const int glo = lo / chunk, ghi = (hi - 1) / chunk;
+  int glo, ghi;
+  if (bs) {
+    const int nbw = (bs[gridDim.y] + G - 1) / G;
+    glo = bs[l] / nbw;
+    ghi = (bs[l + 1] - 1) / nbw;
+  } else {
+    glo = seg[l] / chunk;
+    ghi = (seg[l + 1] - 1) / chunk;
+  }
   const size_t R = (size_t)s * d;
   const size_t q = (size_t)blockIdx.x * NT + threadIdx.x;
   if (q < R) {
@@ -2654,6 +2932,13 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
     return D2MI_OK;
   };
   // COST_TABLE support update: numerators from X, then supports
+  // FP32 stored-cost phases with s, d multiples of 4: step 5 numerators over
+  // the label-sorted columns packed into blocks (k_badmm_supp_dma), from a
+  // descriptor table built after the label sort; D2MI_SUPP_PER_OBJECT=1 keeps
+  // the per-object blocks of k_badmm_supp_mfma
+  bool supp_dma = false;
+  int supp_tpw = 0;
+  const int supp_G = std::min(tab_G, ctx->num_cu * 2);  // LDS: two workgroups per CU
   auto update_supports = [&](hipStream_t S, int *seg, int *flags) -> int {
     D2MI_TRY(ctx->rec2.ensure(sizeof(double) * (size_t)(tab_G + K) * s * dsup));
     SuppArgs<T> sa2{};
@@ -2663,7 +2948,20 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
     const int G = (N + tab_chunk - 1) / tab_chunk;
     // FP32: matrix cores; FP64 (and s > 64): FP64 FMAs
     if constexpr (!F64) {
-      if (s <= 64 && !getenv_flag("D2MI_SUPP_VALU")) {
+      if (supp_dma) {
+        SuppDmaArgs da{};
+        da.s = s; da.d = dsup; da.G = supp_G;
+        da.desc = ctx->sdesc.as<int>(); da.blab = ctx->sblab.as<int>();
+        da.nblk = ctx->sbs.as<int>() + K; da.X = xout; da.W = word ? vocab : supp;
+        da.zeros = ctx->szero.as<float>(); da.rec = ctx->rec2.as<double>();
+        switch (supp_tpw) {
+          case 1: k_badmm_supp_dma<1><<<supp_G, SD_NT, 0, S>>>(da); break;
+          case 2: k_badmm_supp_dma<2><<<supp_G, SD_NT, 0, S>>>(da); break;
+          case 3: k_badmm_supp_dma<3><<<supp_G, SD_NT, 0, S>>>(da); break;
+          case 4: k_badmm_supp_dma<4><<<supp_G, SD_NT, 0, S>>>(da); break;
+          default: k_badmm_supp_dma<5><<<supp_G, SD_NT, 0, S>>>(da); break;
+        }
+      } else if (s <= 64 && !getenv_flag("D2MI_SUPP_VALU")) {
         if (word) k_badmm_supp_mfma<true><<<G, NT, 0, S>>>(sa2);
         else k_badmm_supp_mfma<false><<<G, NT, 0, S>>>(sa2);
       } else if (word) {
@@ -2677,7 +2975,8 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
       k_badmm_supp_num<T, false><<<G, NT, 0, S>>>(sa2);
     }
     k_badmm_supp_fin<T><<<dim3((unsigned)((s * dsup + NT - 1) / NT), K), NT, 0, S>>>(
-        s, dsup, tab_chunk, seg, ctx->rec2.as<double>(),
+        s, dsup, tab_chunk, seg, supp_dma ? ctx->sbs.as<int>() : nullptr, supp_G,
+        ctx->rec2.as<double>(),
                                          ctx->rxbuf.as<double>(), p->cs64.as<double>(), csT, flags);
     D2MI_HIP(hipGetLastError());
     return D2MI_OK;
@@ -2715,6 +3014,38 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
   if (hflags & FLAG_EMPTY) {
     d2mi_set_error("a cluster has no object (the reference asserts, centroid_Bregman.c:144)");
     return D2MI_EEMPTY;
+  }
+  if constexpr (CM == COST_TABLE && !F64) {
+    supp_dma = s <= 64 && s % 4 == 0 && dsup % 4 == 0 && p->col < (size_t)INT32_MAX &&
+               !getenv_flag("D2MI_SUPP_VALU") && !getenv_flag("D2MI_SUPP_PER_OBJECT");
+    if (supp_dma) {
+      const int sweeps = (dsup + 64 * SD_TPW_MAX - 1) / (64 * SD_TPW_MAX);
+      supp_tpw = ((dsup + sweeps - 1) / sweeps + 63) / 64;
+      const size_t nblk_max = p->col / SD_JB + (size_t)K;  // sum over labels of ceil(cols / SD_JB)
+      D2MI_TRY(ctx->sdesc.ensure(sizeof(int) * 32 * nblk_max));
+      D2MI_TRY(ctx->sblab.ensure(sizeof(int) * nblk_max));
+      D2MI_TRY(ctx->sbs.ensure(sizeof(int) * ((size_t)K + 1)));
+      D2MI_TRY(ctx->scum.ensure(sizeof(long long) * ((size_t)N + 1)));
+      D2MI_TRY(ctx->scbt.ensure(sizeof(long long) * (size_t)nb));
+      D2MI_TRY(ctx->szero.ensure(sizeof(float) * 64));
+      const int *perm = ctx->perm.as<int>();
+      long long *scum = ctx->scum.as<long long>();
+      int *sbs = ctx->sbs.as<int>();
+      k_cumw_local<<<nb, NT, 0, S>>>(perm, p->p_str.as<int>(), N, scum, ctx->scbt.as<long long>());
+      k_cumw_carry<<<nb, NT, 0, S>>>(N, ctx->scbt.as<long long>(), scum);
+      k_supp_blocks<<<1, 64, 0, S>>>(K, seg, scum, sbs);
+      if (word)
+        k_supp_desc<true><<<N, 64, 0, S>>>(perm, ctx->label.as<int>(), p->p_str.as<int>(),
+                                           p->cum.as<long long>(), sym, seg, scum, sbs,
+                                           ctx->sdesc.as<int>());
+      else
+        k_supp_desc<false><<<N, 64, 0, S>>>(perm, ctx->label.as<int>(), p->p_str.as<int>(),
+                                            p->cum.as<long long>(), sym, seg, scum, sbs,
+                                            ctx->sdesc.as<int>());
+      k_supp_pad<<<K, 64, 0, S>>>(seg, scum, sbs, ctx->sdesc.as<int>(), ctx->sblab.as<int>());
+      k_zero_i32<<<1, 64, 0, S>>>(ctx->szero.as<int>(), 64);
+      D2MI_HIP(hipGetLastError());
+    }
   }
 
   // ---- 2. setup

@@ -115,7 +115,8 @@ extern "C" int d2mi_destroy(d2mi_ctx *ctx) {
                    &ctx->jobs, &ctx->lpstat, &ctx->cnt, &ctx->mind2, &ctx->red,
                    &ctx->rec2, &ctx->rxbuf, &ctx->bkey, &ctx->permk, &ctx->blkcnt2,
                    &ctx->segk, &ctx->wtab, &ctx->wnorm, &ctx->cs16, &ctx->tuneZ, &ctx->tuneY,
-                   &ctx->rowsum};
+                   &ctx->rowsum, &ctx->sdesc, &ctx->sblab, &ctx->sbs, &ctx->scum,
+                   &ctx->scbt, &ctx->szero};
   for (DevMem *m : all) m->release();
   for (auto &e : ctx->ev)
     if (e) (void)hipEventDestroy(e);

@@ -60,6 +60,36 @@ def test_stored_cost_update_matches_oracle(engine, kind):
     assert rel(c32.p_supp, c64.p_supp) < 1e-4
 
 
+@pytest.mark.parametrize("shape", ["word_d24_s8", "euclid_d64_s8", "word_d300_s64", "word_d100_s32"])
+def test_packed_support_numerators_equal_per_object(engine, shape):
+    """Step 5 numerators (centroid_Bregman.c:199-212) over the label-sorted
+    columns packed into blocks, with their rows copied straight into LDS,
+    against the per-object blocks of the earlier kernel
+    (D2MI_SUPP_PER_OBJECT=1): both FP32 matrix-core sums of the same
+    products, grouped differently, so they agree to FP32 rounding.  Shapes
+    give 1, 2 and 5 output tiles per wave (d = 24 and 64, 100, 300)."""
+    say(f"packed vs per-object support numerators: {shape}")
+    if shape == "euclid_d64_s8":
+        data, ph, c0, lab, lsw = case("euclid_d64")
+    else:
+        d, s = {"word_d24_s8": (24, 8), "word_d300_s64": (300, 64), "word_d100_s32": (100, 32)}[shape]
+        data, _ = synth.word_embed_docs(n=500, V=1500, d=d, m_lo=5, m_hi=120, seed=31, s=s)
+        ph = data.phases[0]
+        c0 = O.centroid_rands(ph, 4, s, seed=1)
+        lab = (np.arange(ph.size) % 4).astype(np.int32)
+        lsw = np.ones(ph.size, np.int8)
+    opts = BADMMOptions(10, 2.0, 3)
+    sa, ca = device_update(engine, data, lab, lsw, c0, opts, 32)
+    os.environ["D2MI_SUPP_PER_OBJECT"] = "1"
+    try:
+        sb, cb = device_update(engine, data, lab, lsw, c0, opts, 32)
+    finally:
+        os.environ.pop("D2MI_SUPP_PER_OBJECT", None)
+    assert rel(ca.p_supp, cb.p_supp) < 1e-5
+    assert rel(ca.p_w, cb.p_w) < 1e-5
+    assert abs(sa["obj"] - sb["obj"]) <= 1e-6 * abs(sb["obj"])
+
+
 def test_stored_and_fused_costs_agree(engine):
     """The same Euclidean d = 3 update with costs recomputed in the plan
     kernel and with the stored table (D2MI_BADMM_TABLE=1)."""
