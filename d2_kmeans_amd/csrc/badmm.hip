@@ -2063,10 +2063,11 @@ __device__ __forceinline__ void lds_copy4(const void *gsrc, void *lds) {
 // MFMA accumulators; a label change flushes them into the FP64 record
 // (g, l).  Needs s % 4 == 0, s <= 64 and d % 4 == 0: rows past s and
 // columns past d are read from row 0 or column 0 (finite) and never stored.
-template <int TPW>
-__global__ __launch_bounds__(SD_NT, 2) void k_badmm_supp_dma(SuppDmaArgs a) {
+template <int TPW, int NS>
+__global__ __launch_bounds__(SD_NT, NS == 2 ? 3 : 2) void k_badmm_supp_dma(SuppDmaArgs a) {
+  static_assert(NS == 2 || NS == 3, "two or three LDS stages");
   constexpr int TD = 64 * TPW, XST = SD_JB * 64, STG = XST + SD_JB * TD;
-  __shared__ __align__(16) float St[3 * STG];  // stage i % 3 at St + (i % 3) STG
+  __shared__ __align__(16) float St[NS * STG];  // stage i % NS at St + (i % NS) STG
   __shared__ __align__(16) int Dsc[SD_DSLOTS][32];
   typedef float v16f __attribute__((ext_vector_type(16)));
   const int tid = threadIdx.x, lane = tid & 63;
@@ -2104,7 +2105,7 @@ __global__ __launch_bounds__(SD_NT, 2) void k_badmm_supp_dma(SuppDmaArgs a) {
       }
     }
   };
-  auto stage_of = [&](int i) __attribute__((always_inline)) -> float * { return St + (i % 3) * STG; };
+  auto stage_of = [&](int i) __attribute__((always_inline)) -> float * { return St + (i % NS) * STG; };
   auto issue_desc = [&](int j) __attribute__((always_inline)) {  // block b0 + j: wave w copies descriptor ints 8 w .. 8 w + 7
     if (lane < 8) lds_copy4(a.desc + (size_t)(b0 + j) * 32 + 8 * w + lane, &Dsc[j % SD_DSLOTS][8 * w]);
   };
@@ -2139,16 +2140,18 @@ __global__ __launch_bounds__(SD_NT, 2) void k_badmm_supp_dma(SuppDmaArgs a) {
       }
     }
   };
-  // block i's copies have landed once at most the copies issued after them
-  // are outstanding: block i + 1's data and block i + 3's descriptors (one
-  // counter orders them; stores issued in between only make the wait
-  // stricter); then a barrier, after which every wave is also done with
-  // block i - 1's stage and descriptor slot, refilled below
+  // Step i issues block i + NS + 1's descriptors, then block i + NS - 1's
+  // data.  Block i's copies have landed once at most the copies issued after
+  // them are outstanding -- with three stages, block i + 1's data and block
+  // i + 3's descriptors; with two, none (one counter orders them; stores
+  // issued in between only make the wait stricter) -- and then a barrier,
+  // after which every wave is also done with block i - 1's stage and
+  // descriptor slot, refilled below.
   constexpr int PER_BLOCK = 1 + TPW;  // data copies per wave per block
   auto wait_block = [&](int i) __attribute__((always_inline)) {
     asm volatile("" ::: "memory");
-    if (i + 3 < nb) __builtin_amdgcn_s_waitcnt(0x0070 | (PER_BLOCK + 1));  // vmcnt(n) lgkmcnt(0)
-    else if (i + 1 < nb) __builtin_amdgcn_s_waitcnt(0x0070 | PER_BLOCK);
+    if (NS == 3 && i + 3 < nb) __builtin_amdgcn_s_waitcnt(0x0070 | (PER_BLOCK + 1));  // vmcnt(n) lgkmcnt(0)
+    else if (NS == 3 && i + 1 < nb) __builtin_amdgcn_s_waitcnt(0x0070 | PER_BLOCK);
     else __builtin_amdgcn_s_waitcnt(0x0070);
     __builtin_amdgcn_s_barrier();
     asm volatile("" ::: "memory");
@@ -2163,14 +2166,14 @@ __global__ __launch_bounds__(SD_NT, 2) void k_badmm_supp_dma(SuppDmaArgs a) {
     __builtin_amdgcn_s_waitcnt(0x0070);
     __builtin_amdgcn_s_barrier();
     asm volatile("" ::: "memory");
-    if (nb > 2) issue_desc(2);
-    issue_data(0, t0);
-    if (nb > 3) issue_desc(3);
-    if (nb > 1) issue_data(1, t0);
+    for (int j = 1 - NS; j < 0; ++j) {  // the steps before block 0's
+      if (j + NS + 1 < nb) issue_desc(j + NS + 1);
+      if (j + NS - 1 < nb) issue_data(j + NS - 1, t0);
+    }
     for (int i = 0; i < nb; ++i) {
       wait_block(i);
-      if (i + 4 < nb) issue_desc(i + 4);
-      if (i + 2 < nb) issue_data(i + 2, t0);
+      if (i + NS + 1 < nb) issue_desc(i + NS + 1);
+      if (i + NS - 1 < nb) issue_data(i + NS - 1, t0);
       const int l = const_load(a.blab + b0 + i);
       if (l != cur) {
         if (cur >= 0) flush(cur, t0);
@@ -2938,7 +2941,12 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
   // the per-object blocks of k_badmm_supp_mfma
   bool supp_dma = false;
   int supp_tpw = 0;
-  const int supp_G = std::min(tab_G, ctx->num_cu * 2);  // LDS: two workgroups per CU
+  // LDS stages: 3 (two workgroups per CU) or 2 (three; D2MI_SUPP_STAGES=2)
+  const int supp_ns = [] {
+    const char *e = getenv("D2MI_SUPP_STAGES");
+    return e && atoi(e) == 2 ? 2 : 3;
+  }();
+  const int supp_G = std::min(tab_G, ctx->num_cu * (supp_ns == 2 ? 3 : 2));
   auto update_supports = [&](hipStream_t S, int *seg, int *flags) -> int {
     D2MI_TRY(ctx->rec2.ensure(sizeof(double) * (size_t)(tab_G + K) * s * dsup));
     SuppArgs<T> sa2{};
@@ -2954,13 +2962,18 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
         da.desc = ctx->sdesc.as<int>(); da.blab = ctx->sblab.as<int>();
         da.nblk = ctx->sbs.as<int>() + K; da.X = xout; da.W = word ? vocab : supp;
         da.zeros = ctx->szero.as<float>(); da.rec = ctx->rec2.as<double>();
-        switch (supp_tpw) {
-          case 1: k_badmm_supp_dma<1><<<supp_G, SD_NT, 0, S>>>(da); break;
-          case 2: k_badmm_supp_dma<2><<<supp_G, SD_NT, 0, S>>>(da); break;
-          case 3: k_badmm_supp_dma<3><<<supp_G, SD_NT, 0, S>>>(da); break;
-          case 4: k_badmm_supp_dma<4><<<supp_G, SD_NT, 0, S>>>(da); break;
-          default: k_badmm_supp_dma<5><<<supp_G, SD_NT, 0, S>>>(da); break;
-        }
+        auto go = [&](auto ns_c) {
+          constexpr int NSc = decltype(ns_c)::value;
+          switch (supp_tpw) {
+            case 1: k_badmm_supp_dma<1, NSc><<<supp_G, SD_NT, 0, S>>>(da); break;
+            case 2: k_badmm_supp_dma<2, NSc><<<supp_G, SD_NT, 0, S>>>(da); break;
+            case 3: k_badmm_supp_dma<3, NSc><<<supp_G, SD_NT, 0, S>>>(da); break;
+            case 4: k_badmm_supp_dma<4, NSc><<<supp_G, SD_NT, 0, S>>>(da); break;
+            default: k_badmm_supp_dma<5, NSc><<<supp_G, SD_NT, 0, S>>>(da); break;
+          }
+        };
+        if (supp_ns == 2) go(std::integral_constant<int, 2>());
+        else go(std::integral_constant<int, 3>());
       } else if (s <= 64 && !getenv_flag("D2MI_SUPP_VALU")) {
         if (word) k_badmm_supp_mfma<true><<<G, NT, 0, S>>>(sa2);
         else k_badmm_supp_mfma<false><<<G, NT, 0, S>>>(sa2);

@@ -60,15 +60,17 @@ def test_stored_cost_update_matches_oracle(engine, kind):
     assert rel(c32.p_supp, c64.p_supp) < 1e-4
 
 
+@pytest.mark.parametrize("stages", ["3", "2"])
 @pytest.mark.parametrize("shape", ["word_d24_s8", "euclid_d64_s8", "word_d300_s64", "word_d100_s32"])
-def test_packed_support_numerators_equal_per_object(engine, shape):
+def test_packed_support_numerators_equal_per_object(engine, shape, stages):
     """Step 5 numerators (centroid_Bregman.c:199-212) over the label-sorted
     columns packed into blocks, with their rows copied straight into LDS,
     against the per-object blocks of the earlier kernel
     (D2MI_SUPP_PER_OBJECT=1): both FP32 matrix-core sums of the same
     products, grouped differently, so they agree to FP32 rounding.  Shapes
-    give 1, 2 and 5 output tiles per wave (d = 24 and 64, 100, 300)."""
-    say(f"packed vs per-object support numerators: {shape}")
+    give 1, 2 and 5 output tiles per wave (d = 24 and 64, 100, 300); the
+    blocks go through three LDS stages, or two (D2MI_SUPP_STAGES=2)."""
+    say(f"packed vs per-object support numerators: {shape}, {stages} stages")
     if shape == "euclid_d64_s8":
         data, ph, c0, lab, lsw = case("euclid_d64")
     else:
@@ -79,7 +81,11 @@ def test_packed_support_numerators_equal_per_object(engine, shape):
         lab = (np.arange(ph.size) % 4).astype(np.int32)
         lsw = np.ones(ph.size, np.int8)
     opts = BADMMOptions(10, 2.0, 3)
-    sa, ca = device_update(engine, data, lab, lsw, c0, opts, 32)
+    os.environ["D2MI_SUPP_STAGES"] = stages
+    try:
+        sa, ca = device_update(engine, data, lab, lsw, c0, opts, 32)
+    finally:
+        os.environ.pop("D2MI_SUPP_STAGES", None)
     os.environ["D2MI_SUPP_PER_OBJECT"] = "1"
     try:
         sb, cb = device_update(engine, data, lab, lsw, c0, opts, 32)
