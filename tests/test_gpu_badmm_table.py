@@ -96,6 +96,30 @@ def test_packed_support_numerators_equal_per_object(engine, shape, stages):
     assert abs(sa["obj"] - sb["obj"]) <= 1e-6 * abs(sb["obj"])
 
 
+@pytest.mark.parametrize("s", [8, 12])
+def test_word_gather_forms_agree(engine, s):
+    """Cost blocks gathered from the pdist2 table 16 bytes per thread (s a
+    multiple of 4) are the same copy as the element-wise gather
+    (D2MI_GATHER_SCALAR=1): an update from the same state ends bit for bit
+    the same."""
+    say(f"word gather forms, s={s}")
+    data, _ = synth.word_embed_docs(n=400, V=800, d=24, m_lo=5, m_hi=90, seed=37, s=s)
+    ph = data.phases[0]
+    c0 = O.centroid_rands(ph, 4, s, seed=2)
+    lab = (np.arange(ph.size) % 4).astype(np.int32)
+    lsw = np.ones(ph.size, np.int8)
+    opts = BADMMOptions(8, 2.0, 3)
+    sa, ca = device_update(engine, data, lab, lsw, c0, opts, 32)
+    os.environ["D2MI_GATHER_SCALAR"] = "1"
+    try:
+        sb, cb = device_update(engine, data, lab, lsw, c0, opts, 32)
+    finally:
+        os.environ.pop("D2MI_GATHER_SCALAR", None)
+    np.testing.assert_array_equal(ca.p_supp, cb.p_supp)
+    np.testing.assert_array_equal(ca.p_w, cb.p_w)
+    assert sa["obj"] == sb["obj"]
+
+
 def test_stored_and_fused_costs_agree(engine):
     """The same Euclidean d = 3 update with costs recomputed in the plan
     kernel and with the stored table (D2MI_BADMM_TABLE=1)."""
