@@ -2331,6 +2331,30 @@ __global__ __launch_bounds__(NT) void k_badmm_setup(SetupArgs<T> a) {
   if (tid == 0) a.part[blockIdx.x] = csum;
 }
 
+// Ground-cost sum of a stored cost table (rho, :109): the table holds every
+// object's s x m block back to back, so the sum is a flat reduction, 16 bytes
+// (FP32) or 32 bytes (FP64) per thread per step, one FP64 partial per block.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_abs_sum(const T *__restrict__ c, size_t n,
+                                                double *__restrict__ part) {
+  __shared__ double red[8];
+  double v = 0.0;
+  const size_t nq = n / 4;
+  for (size_t q = blockIdx.x * (size_t)NT + threadIdx.x; q < nq; q += (size_t)gridDim.x * NT) {
+    if constexpr (std::is_same<T, float>::value) {
+      const float4 x = ((const float4 *)c)[q];
+      v += (fabs((double)x.x) + fabs((double)x.y)) + (fabs((double)x.z) + fabs((double)x.w));
+    } else {
+      const double2 x = ((const double2 *)c)[2 * q], y = ((const double2 *)c)[2 * q + 1];
+      v += (fabs(x.x) + fabs(x.y)) + (fabs(y.x) + fabs(y.y));
+    }
+  }
+  if (blockIdx.x == 0)
+    for (size_t i = nq * 4 + threadIdx.x; i < n; i += NT) v += fabs((double)c[i]);
+  v = block_sum(v, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = v;
+}
+
 // Ground-cost sum of a Euclidean phase (rho, :109) from each object's mean
 // support point mu_i and spread sum_j |x_j - mu_i|^2 (computed at upload):
 // sum_k sum_j |c_k - x_j|^2 = m_i sum_k |c_k - mu_i|^2 + s spread_i, every
@@ -3145,6 +3169,9 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
       k_badmm_rho_hist<<<nparts, NT, 0, S>>>(N, p->p_str.as<int>(), p->cum.as<long long>(), sym,
                                              sparse ? 1 : 0, ctx->rowsum.as<double>(),
                                              ctx->part.as<double>());
+    } else if (variant == D2MI_BADMM_ENGINE && CM == COST_TABLE && !getenv_flag("D2MI_RHO_DIRECT")) {
+      nparts = ctx->num_cu * 4;
+      k_abs_sum<T><<<nparts, NT, 0, S>>>(ctab, (size_t)s * p->col, ctx->part.as<double>());
     } else {
       k_badmm_setup<T, CM><<<setup_blocks, NT, lds, S>>>(sa);
     }

@@ -96,6 +96,29 @@ def test_packed_support_numerators_equal_per_object(engine, shape, stages):
     assert abs(sa["obj"] - sb["obj"]) <= 1e-6 * abs(sb["obj"])
 
 
+@pytest.mark.parametrize("precision", [64, 32])
+def test_rho_from_flat_table_sum(engine, precision):
+    """rho (centroid_Bregman.c:109) of a word-embedding phase from one flat
+    sum over the cost table (every object's block back to back) against the
+    object-by-object sum of the setup kernel (D2MI_RHO_DIRECT=1): the same
+    terms in another order, so 1e-13 relative; FP64 also against the oracle.
+    (Euclidean phases take rho from object moments instead.)"""
+    say(f"rho from the flat table sum: FP{precision}")
+    data, ph, c0, lab, lsw = case("word_embed")
+    opts = BADMMOptions(2, 2.0, 5)
+    sa, _ = device_update(engine, data, lab, lsw, c0, opts, precision)
+    os.environ["D2MI_RHO_DIRECT"] = "1"
+    try:
+        sb, _ = device_update(engine, data, lab, lsw, c0, opts, precision)
+    finally:
+        os.environ.pop("D2MI_RHO_DIRECT", None)
+    assert abs(sa["rho"] - sb["rho"]) <= 1e-13 * sb["rho"], (sa["rho"], sb["rho"])
+    if precision == 64:
+        co = c0.copy()
+        ro = O.badmm(ph, lab, np.ones(ph.size, np.int8), co, np.zeros(c0.s * ph.col), opts)
+        assert abs(sa["rho"] - ro["rho"]) <= 1e-12 * ro["rho"]
+
+
 @pytest.mark.parametrize("s", [8, 12])
 def test_word_gather_forms_agree(engine, s):
     """Cost blocks gathered from the pdist2 table 16 bytes per thread (s a
