@@ -137,6 +137,11 @@ struct PlanLds {
   }
 };
 
+// object o = perm[p], its label, column count and first column
+struct alignas(16) ObjMeta {
+  int o, l, m, off;
+};
+
 template <typename T>
 struct PlanArgs {
   int N, s, d, K, sp, m_max, chunk, R, V;
@@ -162,6 +167,7 @@ struct PlanArgs {
   T *__restrict__ z0g;              // HBM-scratch kernel, residual passes: Z before the pass
   double *__restrict__ rec;         // [(G + K) * R]
   double *__restrict__ resid_part;  // [G * 3]
+  const struct ObjMeta *__restrict__ pm;  // or null: per position of perm, its object's metadata
 };
 
 // ground cost of element (k, j) before the division by rho; DIM > 0 fixes
@@ -1053,6 +1059,14 @@ void k_badmm_plan_reg(PlanArgs<T> a) {
   // previous pass wrote last, which are still in the memory-side cache
   const int np = p1 - p0, rev = a.rev;
   if (PF && np > 0) prefetch(rev ? p1 - 1 : p0);
+  // with pm, each object's metadata arrives one object ahead
+  // four adjacent scalar loads: one 16-byte load of the entry
+  auto meta_load = [&](int q) -> ObjMeta {
+    const int *e = (const int *)(a.pm + q);
+    return ObjMeta{const_load(e), const_load(e + 1), const_load(e + 2), const_load(e + 3)};
+  };
+  ObjMeta mt_next{0, 0, 0, 0};
+  if (a.pm && np > 0) mt_next = meta_load(rev ? p1 - 1 : p0);
 
   for (int i = 0; i < np; ++i) {
     const int p = rev ? p1 - 1 - i : p0 + i;
@@ -1065,10 +1079,21 @@ void k_badmm_plan_reg(PlanArgs<T> a) {
     const int bo = resid ? (i & 1) : 0;
     T *xo = xs + (size_t)bo * a.m_max * (d > 0 ? d : 1);
     int *io = ids + bo * a.m_max;
-    const int o = const_load(a.perm + p);
-    const int l = const_load(a.label + o);
-    const int m = const_load(a.p_str + o);
-    const long long off = const_load(a.cum + o);
+    int o, l, m;
+    long long off;
+    if (a.pm) {
+      const ObjMeta mt = mt_next;
+      if (i + 1 < np) mt_next = meta_load(rev ? p - 1 : p + 1);
+      o = mt.o;
+      l = mt.l;
+      m = mt.m;
+      off = (unsigned int)mt.off;
+    } else {
+      o = const_load(a.perm + p);
+      l = const_load(a.label + o);
+      m = const_load(a.p_str + o);
+      off = const_load(a.cum + o);
+    }
     const size_t base = (size_t)s * off;
     const bool reinit = first && (a.force_init || const_load(a.lsw + o) == 1);
     // this object's plan: element (k, j) at base + k + j s; the thread's
@@ -2641,6 +2666,21 @@ __global__ void k_zero_f64(double *p, size_t n) {
     p[i] = 0.0;
 }
 
+// Object metadata in the plan kernels' order: pm[p] = {object, label, column
+// count, first column} of perm[p], one 16-byte scalar load per object instead
+// of a load of perm[p] and three loads that wait for it
+__global__ __launch_bounds__(NT) void k_plan_meta(const int *__restrict__ perm, int n,
+                                                  const int *__restrict__ label,
+                                                  const int *__restrict__ p_str,
+                                                  const long long *__restrict__ cum,
+                                                  ObjMeta *__restrict__ pm) {
+  const int p = blockIdx.x * NT + threadIdx.x;
+  if (p < n) {
+    const int o = perm[p];
+    pm[p] = ObjMeta{o, label[o], p_str[o], (int)cum[o]};
+  }
+}
+
 __global__ void k_zero_i32(int *p, size_t n) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x)
@@ -3352,6 +3392,17 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
       bk[0].start = 0;
       bk[0].count = N;
     }
+    // per-object metadata in the plan order (D2MI_PLAN_NO_PMETA=1: the kernels
+    // look it up through perm)
+    const ObjMeta *pmeta = nullptr;
+    if (p->col < (size_t)INT32_MAX && !getenv_flag("D2MI_PLAN_NO_PMETA")) {
+      D2MI_TRY(ctx->pmeta.ensure(sizeof(ObjMeta) * (size_t)N));
+      k_plan_meta<<<(N + NT - 1) / NT, NT, 0, S>>>(permk, N, ctx->label.as<int>(),
+                                                  p->p_str.as<int>(), p->cum.as<long long>(),
+                                                  ctx->pmeta.as<ObjMeta>());
+      D2MI_HIP(hipGetLastError());
+      pmeta = ctx->pmeta.as<ObjMeta>();
+    }
 
     // ---- kernel forms: which fit a class (LDS, columns per thread), the
     // rule-based choice, and the tuner's choice where it has one
@@ -3504,6 +3555,7 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
       pa.xg = B.gm ? p->Xout.as<T>() : nullptr; pa.z0g = B.gm ? p->Z0.as<T>() : nullptr;
       pa.rec = ctx->rec.as<double>() + (size_t)B.rec_off * R;
       pa.first = first; pa.force_init = fresh ? 1 : 0; pa.lsw = ctx->lsw.as<char>();
+      pa.pm = pmeta ? pmeta + B.start : nullptr;
       return pa;
     };
     if (any_gm) {

@@ -116,7 +116,7 @@ extern "C" int d2mi_destroy(d2mi_ctx *ctx) {
                    &ctx->rec2, &ctx->rxbuf, &ctx->bkey, &ctx->permk, &ctx->blkcnt2,
                    &ctx->segk, &ctx->wtab, &ctx->wnorm, &ctx->cs16, &ctx->tuneZ, &ctx->tuneY,
                    &ctx->rowsum, &ctx->sdesc, &ctx->sblab, &ctx->sbs, &ctx->scum,
-                   &ctx->scbt, &ctx->szero};
+                   &ctx->scbt, &ctx->szero, &ctx->pmeta};
   for (DevMem *m : all) m->release();
   for (auto &e : ctx->ev)
     if (e) (void)hipEventDestroy(e);

@@ -99,6 +99,7 @@ struct d2mi_ctx {
   DevMem rec2, rxbuf;           // support-numerator records, X row sums per label
   DevMem bkey, permk, blkcnt2, segk;  // B-ADMM order by (object-size bucket, label)
   DevMem sdesc, sblab, sbs, scum, scbt, szero;  // step 5 over packed columns: block descriptors
+  DevMem pmeta;                       // B-ADMM: {object, label, m, first column} in plan order
   DevMem wtab, wnorm;           // word embeddings: pdist2(vocab, centroid supports), norms
   DevMem cs16;                  // BF16 centroid supports for the BF16 pdist2
   DevMem rowsum;                // histogram phases: ground-cost line sums for rho
