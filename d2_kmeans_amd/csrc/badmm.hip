@@ -873,6 +873,12 @@ void k_badmm_plan64(PlanArgs<T> a) {
 #ifndef D2MI_PLANREG_RESID_WAVES
 #define D2MI_PLANREG_RESID_WAVES 4  // residual passes: 4 waves per SIMD (a small spill)
 #endif
+#ifndef D2MI_PLANREG_SETPRIO
+#define D2MI_PLANREG_SETPRIO 2  // wave priority while an object's loads are issued (0: none)
+#endif
+#ifndef D2MI_PLANREG_SETPRIO_ST
+#define D2MI_PLANREG_SETPRIO_ST 0  // wave priority while an object's stores are issued
+#endif
 #ifndef D2MI_PLANREG_MID_CAP
 #define D2MI_PLANREG_MID_CAP 1  // FP32 support-update passes share the lean budget
 #endif
@@ -1100,7 +1106,9 @@ void k_badmm_plan_reg(PlanArgs<T> a) {
     // column slot c is voff + c * cstride bytes from the base
     const BufRes rz = buf_res(a.Z + base), ry = buf_res(a.Y + base);
     const int voff = (ke + jb * s) * (int)sizeof(T), cstride = cpi * s * (int)sizeof(T);
-    // requests first: plan elements, weights, supports or bin ids
+    // requests first: plan elements, weights, supports or bin ids, at raised
+    // wave priority (D2MI_PLANREG_SETPRIO) so they leave before other waves' work
+    if (D2MI_PLANREG_SETPRIO) __builtin_amdgcn_s_setprio(D2MI_PLANREG_SETPRIO);
     T zr[NC], yr[NC], xr[NC], cr[NC];
     if constexpr (PF) {
       full_sync();  // every wave's share of this object's copy has landed
@@ -1144,6 +1152,7 @@ void k_badmm_plan_reg(PlanArgs<T> a) {
         if (q < m * d) xpre[r] = a.supp[(size_t)off * d + q];
       }
     }
+    if (D2MI_PLANREG_SETPRIO) __builtin_amdgcn_s_setprio(0);
     if (l != cur) {
       __syncthreads();  // every wave is done with the previous label
       if (cur >= 0)
@@ -1340,6 +1349,7 @@ void k_badmm_plan_reg(PlanArgs<T> a) {
       }
 #pragma unroll
       for (int c = 0; c < NC; ++c) asm volatile("" : "+v"(zr[c]), "+v"(yr[c]));
+      if (D2MI_PLANREG_SETPRIO_ST) __builtin_amdgcn_s_setprio(D2MI_PLANREG_SETPRIO_ST);
 #pragma unroll
       for (int c = 0; c < NC; ++c) {
         if (jb + c * cpi < m) {
@@ -1347,6 +1357,7 @@ void k_badmm_plan_reg(PlanArgs<T> a) {
           buf_store<T>(yr[c], ry, voff, c * cstride);
         }
       }
+      if (D2MI_PLANREG_SETPRIO_ST) __builtin_amdgcn_s_setprio(0);
     }
   }
   __syncthreads();
