@@ -1429,6 +1429,8 @@ __global__ __launch_bounds__(TALL_NT) void k_badmm_plan_tall(PlanArgs<T> a) {
       acc = 0.0;
       cur = l;
     }
+    // the object's loads at raised wave priority, as in the register form
+    if (D2MI_PLANREG_SETPRIO) __builtin_amdgcn_s_setprio(D2MI_PLANREG_SETPRIO);
     const T cwk = a.cw[(size_t)l * s + k];
     const BufRes rz = buf_res(a.Z + base), ry = buf_res(a.Y + base);
     T zr[MC], yr[MC];
@@ -1438,6 +1440,7 @@ __global__ __launch_bounds__(TALL_NT) void k_badmm_plan_tall(PlanArgs<T> a) {
       zr[c] = reinit ? (T)0 : buf_load<T>(rz, voff, cc * cstride);
       yr[c] = first ? (T)0 : buf_load<T>(ry, voff, cc * cstride);
     }
+    if (D2MI_PLANREG_SETPRIO) __builtin_amdgcn_s_setprio(0);
 
     // pass 1 -- step 1a: t = Z exp(-(C + Y)) + 1e-9 (:155-157), C = dist[id_j][k],
     // 8 columns at a time (a scheduling barrier between groups keeps every
