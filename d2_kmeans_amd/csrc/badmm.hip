@@ -876,6 +876,14 @@ void k_badmm_plan64(PlanArgs<T> a) {
 #ifndef D2MI_PLANREG_SETPRIO
 #define D2MI_PLANREG_SETPRIO 2  // wave priority while an object's loads are issued (0: none)
 #endif
+// the short phases every wave then waits for at a barrier run at raised
+// priority too: the column sums, and wave 0's row sums
+#ifndef D2MI_PLANREG_PRIO_P2
+#define D2MI_PLANREG_PRIO_P2 2  // wave priority during the column sums (0: none)
+#endif
+#ifndef D2MI_PLANREG_PRIO_S4
+#define D2MI_PLANREG_PRIO_S4 2  // wave priority of wave 0's row sums (0: none)
+#endif
 #ifndef D2MI_PLANREG_SETPRIO_ST
 #define D2MI_PLANREG_SETPRIO_ST 0  // wave priority while an object's stores are issued
 #endif
@@ -1228,6 +1236,7 @@ void k_badmm_plan_reg(PlanArgs<T> a) {
     }
     __syncthreads();
     // pass 2 -- step 1b: column sums (_dcnorm, :158) -> the column's scale
+    if (D2MI_PLANREG_PRIO_P2) __builtin_amdgcn_s_setprio(D2MI_PLANREG_PRIO_P2);
     for (int j = tid >> 2; j < m; j += NTB / 4) {
       const T *colA = A + j * sp + r0;
       T part = 0;
@@ -1240,6 +1249,7 @@ void k_badmm_plan_reg(PlanArgs<T> a) {
       part += __shfl_xor(part, 2, 64);
       if (q4 == 0) csl[j] = F64 ? part : wl[j] / part;
     }
+    if (D2MI_PLANREG_PRIO_P2) __builtin_amdgcn_s_setprio(0);
     __syncthreads();
     // t (in stZ) is read: the next object's copy can start
     if (PF && i + 1 < np) prefetch(rev ? p - 1 : p + 1);
@@ -1281,6 +1291,7 @@ void k_badmm_plan_reg(PlanArgs<T> a) {
     else __syncthreads();
     // step 4: normalised row sums summed per label (:180-185)
     if (wv == 0) {
+      if (D2MI_PLANREG_PRIO_S4) __builtin_amdgcn_s_setprio(D2MI_PLANREG_PRIO_S4);
       T r = 0;
       if (lane < s) {
         for (int c = 0; c < cpi; ++c) r += rp[c * s + lane];
@@ -1296,6 +1307,7 @@ void k_badmm_plan_reg(PlanArgs<T> a) {
           else acc[s + s * d + lane] += v;
         }
       }
+      if (D2MI_PLANREG_PRIO_S4) __builtin_amdgcn_s_setprio(0);
     }
     if constexpr (PF) lds_sync();
     else __syncthreads();
