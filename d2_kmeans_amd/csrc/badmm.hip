@@ -884,6 +884,9 @@ void k_badmm_plan64(PlanArgs<T> a) {
 #ifndef D2MI_PLANREG_PRIO_S4
 #define D2MI_PLANREG_PRIO_S4 2  // wave priority of wave 0's row sums (0: none)
 #endif
+#ifndef D2MI_PLANREG_PRIO_P4
+#define D2MI_PLANREG_PRIO_P4 2  // wave priority of the last pass's arithmetic (0: none)
+#endif
 #ifndef D2MI_PLANREG_SETPRIO_ST
 #define D2MI_PLANREG_SETPRIO_ST 0  // wave priority while an object's stores are issued
 #endif
@@ -1319,6 +1322,7 @@ void k_badmm_plan_reg(PlanArgs<T> a) {
     // so no barrier follows this pass.  An object's terms are summed in T and
     // added to the FP64 totals once.
     {
+      if (D2MI_PLANREG_PRIO_P4) __builtin_amdgcn_s_setprio(D2MI_PLANREG_PRIO_P4);
       const T gk = g[ke];
       T ob = 0, pr = 0, dr = 0;
 #pragma unroll
@@ -1361,6 +1365,7 @@ void k_badmm_plan_reg(PlanArgs<T> a) {
       }
 #pragma unroll
       for (int c = 0; c < NC; ++c) asm volatile("" : "+v"(zr[c]), "+v"(yr[c]));
+      if (D2MI_PLANREG_PRIO_P4) __builtin_amdgcn_s_setprio(0);
       if (D2MI_PLANREG_SETPRIO_ST) __builtin_amdgcn_s_setprio(D2MI_PLANREG_SETPRIO_ST);
 #pragma unroll
       for (int c = 0; c < NC; ++c) {
