@@ -28,6 +28,7 @@
 #include <type_traits>
 
 #include "d2mi_internal.h"
+#include "lds_copy.h"
 
 namespace {
 
@@ -2123,32 +2124,6 @@ struct SuppDmaArgs {
   const float *__restrict__ zeros;  // 64 zero floats
   double *__restrict__ rec;      // record (g, l) at g + l, s x d each
 };
-
-// One wave's copy from global memory into LDS, issued by hand (the
-// compiler's own form makes every later LDS read wait for all copies in
-// flight): 4 or 16 bytes per lane, lane l's bytes to lds + 4 l or 16 l, lds
-// the same for every lane.  One wait state between the M0 write and the
-// copy.  No register output; the callers count these copies in their own
-// s_waitcnt vmcnt.
-#pragma clang diagnostic push
-#pragma clang diagnostic ignored "-Winline-asm"
-__device__ __forceinline__ void lds_copy16(const void *gsrc, void *lds) {
-  const unsigned int m0 = __builtin_amdgcn_readfirstlane(
-      (unsigned int)(size_t)(__attribute__((address_space(3))) char *)lds);  // wave-uniform
-  asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off"
-               :
-               : "v"(gsrc), "s"(m0)
-               : "memory", "m0");
-}
-__device__ __forceinline__ void lds_copy4(const void *gsrc, void *lds) {
-  const unsigned int m0 = __builtin_amdgcn_readfirstlane(
-      (unsigned int)(size_t)(__attribute__((address_space(3))) char *)lds);  // wave-uniform
-  asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dword %0, off"
-               :
-               : "v"(gsrc), "s"(m0)
-               : "memory", "m0");
-}
-#pragma clang diagnostic pop
 
 // Workgroup g takes an equal run of blocks (so equal work).  Per sweep of
 // 64 TPW output columns, block i goes through three LDS stages: its

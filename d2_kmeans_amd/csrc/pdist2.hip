@@ -33,6 +33,7 @@
 #include <vector>
 
 #include "d2mi_internal.h"
+#include "lds_copy.h"
 #include "transport.h"
 
 namespace {
@@ -352,28 +353,7 @@ __global__ __launch_bounds__(128 * WN, WN == 2 ? 3 : 4) void k_pdist2_bf16(
 // different bank groups.  Rows past P or Q load row P - 1 or Q - 1 (their
 // results are never stored).
 constexpr int DMA_STAGES = 3;
-// One wave's 16-byte-per-lane copy from global memory into LDS (lane l's
-// 16 bytes to lds + 16 l), issued by hand: the compiler's own form of this
-// instruction makes every later LDS read wait for all copies in flight
-// (vmcnt(0)), which would drain the pipeline each slice.  One wait state
-// between the M0 write and the copy that reads it.  No register
-// output; the address registers are read at issue (the compiler's form
-// reuses them right after it too).  The callers count these copies in their
-// own s_waitcnt vmcnt and wait for all of them before any other vector
-// memory access is awaited.
-// (M0 is a reserved register: the compiler notes its clobber without
-// tracking it, and nothing else in these kernels uses M0.)
-#pragma clang diagnostic push
-#pragma clang diagnostic ignored "-Winline-asm"
-__device__ __forceinline__ void dma16(const void *gsrc, __bf16 *lds) {
-  const unsigned int m0 =
-      (unsigned int)(size_t)(__attribute__((address_space(3))) __bf16 *)lds;
-  asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off"
-               :
-               : "v"(gsrc), "s"(m0)
-               : "memory", "m0");
-}
-#pragma clang diagnostic pop
+// The slices are copied with lds_copy16 (lds_copy.h).
 template <int WN>
 __global__ __launch_bounds__(128 * WN, 4) void k_pdist2_bf16_dma(
     const __bf16 *__restrict__ xb, const __bf16 *__restrict__ yb, int P, int Q, int dp,
@@ -424,7 +404,7 @@ __global__ __launch_bounds__(128 * WN, 4) void k_pdist2_bf16_dma(
       const __bf16 *src;
       if (blk < XB) src = xb + (size_t)min(p0 + blk * 16 + lr, P - 1) * dp + k0;
       else src = yb + (size_t)min(q0 + (blk - XB) * 16 + lr, Q - 1) * dp + k0;
-      dma16(src, &St[blk * 16 * BK]);
+      lds_copy16(src, &St[blk * 16 * BK]);
     }
   };
   auto products = [&](auto st_c) {
