@@ -1537,11 +1537,16 @@ __global__ __launch_bounds__(TALL_NT) void k_badmm_plan_tall(PlanArgs<T> a) {
     }
 #pragma unroll
     for (int c = 0; c < MC; ++c) asm volatile("" : "+v"(zr[c]), "+v"(yr[c]));
+    // the column offsets again from opaque copies of the stride and the
+    // column count: otherwise the compiler keeps MC offsets live in scalar
+    // registers from the loads to here, and spills them
+    int cst = cstride, ms = m;
+    asm volatile("" : "+s"(cst), "+s"(ms));
 #pragma unroll
     for (int c = 0; c < MC; ++c) {
-      if (c < m && row) {
-        buf_store<T>(zr[c], rz, voff, c * cstride);
-        buf_store<T>(yr[c], ry, voff, c * cstride);
+      if (c < ms && row) {
+        buf_store<T>(zr[c], rz, voff, c * cst);
+        buf_store<T>(yr[c], ry, voff, c * cst);
       }
     }
   }
