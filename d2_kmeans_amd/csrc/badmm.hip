@@ -1458,6 +1458,10 @@ __global__ __launch_bounds__(TALL_NT) void k_badmm_plan_tall(PlanArgs<T> a) {
       zr[c] = reinit ? (T)0 : buf_load<T>(rz, voff, cc * cstride);
       yr[c] = first ? (T)0 : buf_load<T>(ry, voff, cc * cstride);
     }
+    // the object's bin ids in one vector load, lane j holding column j's
+    // (columns past m repeat the last); each column reads its id out with
+    // v_readlane instead of a scalar load that the next one waits behind
+    const int idv = a.sym[off + min(lane, m - 1)];
     if (D2MI_PLANREG_SETPRIO) __builtin_amdgcn_s_setprio(0);
 
     // pass 1 -- step 1a: t = Z exp(-(C + Y)) + 1e-9 (:155-157), C = dist[id_j][k],
@@ -1466,7 +1470,7 @@ __global__ __launch_bounds__(TALL_NT) void k_badmm_plan_tall(PlanArgs<T> a) {
 #pragma unroll
     for (int c = 0; c < MC; ++c) {
       const int cc = min(c, m - 1);
-      const int id = const_load(a.sym + off + cc);
+      const int id = __builtin_amdgcn_readlane(idv, c);
       const T cost = buf_load<T>(rdist, voff, id * vstride);
       const T z = reinit ? const_load(a.w + off + cc) * cwk : zr[c];
       zr[c] = z * plan_exp<T>(-(scale_cost<T>(cost, rho, inv_rho) + yr[c])) + (T)1e-9;
@@ -1522,7 +1526,7 @@ __global__ __launch_bounds__(TALL_NT) void k_badmm_plan_tall(PlanArgs<T> a) {
         if (resid && c < m) {  // step 6 (:300-305): Z before this pass, cost again
           const T z0 = reinit ? const_load(a.w + off + c) * cwk : buf_load<T>(rz, voff, c * cstride);
           const T cost = scale_cost<T>(
-              buf_load<T>(rdist, voff, const_load(a.sym + off + c) * vstride), rho, inv_rho);
+              buf_load<T>(rdist, voff, __builtin_amdgcn_readlane(idv, c) * vstride), rho, inv_rho);
           if (row) {
             r_obj += (double)cost * (double)xr[c];
             r_pr += fabs((double)xr[c] - (double)z);
