@@ -888,9 +888,6 @@ void k_badmm_plan64(PlanArgs<T> a) {
 #ifndef D2MI_PLANREG_PRIO_P4
 #define D2MI_PLANREG_PRIO_P4 2  // wave priority of the last pass's arithmetic (0: none)
 #endif
-#ifndef D2MI_PLANREG_SETPRIO_ST
-#define D2MI_PLANREG_SETPRIO_ST 0  // wave priority while an object's stores are issued
-#endif
 #ifndef D2MI_PLANREG_MID_CAP
 #define D2MI_PLANREG_MID_CAP 1  // FP32 support-update passes share the lean budget
 #endif
@@ -1367,7 +1364,6 @@ void k_badmm_plan_reg(PlanArgs<T> a) {
 #pragma unroll
       for (int c = 0; c < NC; ++c) asm volatile("" : "+v"(zr[c]), "+v"(yr[c]));
       if (D2MI_PLANREG_PRIO_P4) __builtin_amdgcn_s_setprio(0);
-      if (D2MI_PLANREG_SETPRIO_ST) __builtin_amdgcn_s_setprio(D2MI_PLANREG_SETPRIO_ST);
 #pragma unroll
       for (int c = 0; c < NC; ++c) {
         if (jb + c * cpi < m) {
@@ -1375,7 +1371,6 @@ void k_badmm_plan_reg(PlanArgs<T> a) {
           buf_store<T>(yr[c], ry, voff, c * cstride);
         }
       }
-      if (D2MI_PLANREG_SETPRIO_ST) __builtin_amdgcn_s_setprio(0);
     }
   }
   __syncthreads();
