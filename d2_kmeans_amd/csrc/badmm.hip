@@ -866,10 +866,13 @@ void k_badmm_plan64(PlanArgs<T> a) {
 
 // ------------------------------------ register-resident plan kernel (s <= 64)
 
-// register budget of the lean passes: 6 waves per SIMD with 8 column slots,
-// 5 with 16 (the slots alone take 32 registers)
+// register budget of the lean passes: 6 waves per SIMD with 8 column slots;
+// with 16, 4 for fused Euclidean and stored costs (about 101-128 registers,
+// no spill: faster than 5 waves with 96 registers and 9-28 spilled) and 5 for
+// histogram costs (their lean passes fit 96 registers without a spill)
 #ifndef D2MI_PLANREG_WAVES_PER_EU
-#define D2MI_PLANREG_WAVES_PER_EU(nc) ((nc) <= 8 ? 6 : 5)
+#define D2MI_PLANREG_WAVES_PER_EU(nc, cm) \
+  ((nc) <= 8 ? 6 : ((cm) == COST_HIST || (cm) == COST_SPARSE) ? 5 : 4)
 #endif
 #ifndef D2MI_PLANREG_RESID_WAVES
 #define D2MI_PLANREG_RESID_WAVES 4  // residual passes: 4 waves per SIMD (a small spill)
@@ -1012,7 +1015,7 @@ __global__ __launch_bounds__(NTB)
 __attribute__((amdgpu_waves_per_eu(
     PF ? 4  // the staging arrays (36 KB) allow 4 workgroups per CU
     : MODE == 0 || (MODE == 1 && D2MI_PLANREG_MID_CAP && !std::is_same<T, double>::value)
-        ? D2MI_PLANREG_WAVES_PER_EU(NC)
+        ? D2MI_PLANREG_WAVES_PER_EU(NC, CM)
         : (MODE == 2 && !std::is_same<T, double>::value ? D2MI_PLANREG_RESID_WAVES : 1))))
 void k_badmm_plan_reg(PlanArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem[];
