@@ -140,8 +140,9 @@ struct PlanLds {
 
 // object o = perm[p], its label, column count and first column
 struct alignas(16) ObjMeta {
-  int o, l, m, off;
+  int o, l, m, off;  // m carries the object's label-switch flag in bit 30
 };
+constexpr int META_SWITCHED = 1 << 30;
 
 template <typename T>
 struct PlanArgs {
@@ -1099,21 +1100,26 @@ void k_badmm_plan_reg(PlanArgs<T> a) {
     int *io = ids + bo * a.m_max;
     int o, l, m;
     long long off;
+    bool switched;
     if (a.pm) {
       const ObjMeta mt = mt_next;
       if (i + 1 < np) mt_next = meta_load(rev ? p - 1 : p + 1);
       o = mt.o;
       l = mt.l;
-      m = mt.m;
+      m = mt.m & (META_SWITCHED - 1);
+      switched = (mt.m & META_SWITCHED) != 0;
       off = (unsigned int)mt.off;
     } else {
       o = const_load(a.perm + p);
       l = const_load(a.label + o);
       m = const_load(a.p_str + o);
       off = const_load(a.cum + o);
+      switched = first && const_load(a.lsw + o) == 1;
     }
     const size_t base = (size_t)s * off;
-    const bool reinit = first && (a.force_init || const_load(a.lsw + o) == 1);
+    // the flag comes with the metadata, so no dependent load waits before
+    // the object's plan loads
+    const bool reinit = first && (a.force_init || switched);
     // this object's plan: element (k, j) at base + k + j s; the thread's
     // column slot c is voff + c * cstride bytes from the base
     const BufRes rz = buf_res(a.Z + base), ry = buf_res(a.Y + base);
@@ -2686,11 +2692,12 @@ __global__ __launch_bounds__(NT) void k_plan_meta(const int *__restrict__ perm, 
                                                   const int *__restrict__ label,
                                                   const int *__restrict__ p_str,
                                                   const long long *__restrict__ cum,
+                                                  const char *__restrict__ lsw,
                                                   ObjMeta *__restrict__ pm) {
   const int p = blockIdx.x * NT + threadIdx.x;
   if (p < n) {
     const int o = perm[p];
-    pm[p] = ObjMeta{o, label[o], p_str[o], (int)cum[o]};
+    pm[p] = ObjMeta{o, label[o], p_str[o] | (lsw[o] == 1 ? META_SWITCHED : 0), (int)cum[o]};
   }
 }
 
@@ -3412,7 +3419,7 @@ int run_badmm(d2mi_ctx *ctx, PhaseDev *p, int maxIters, double rhoCoeff, int upd
       D2MI_TRY(ctx->pmeta.ensure(sizeof(ObjMeta) * (size_t)N));
       k_plan_meta<<<(N + NT - 1) / NT, NT, 0, S>>>(permk, N, ctx->label.as<int>(),
                                                   p->p_str.as<int>(), p->cum.as<long long>(),
-                                                  ctx->pmeta.as<ObjMeta>());
+                                                  ctx->lsw.as<char>(), ctx->pmeta.as<ObjMeta>());
       D2MI_HIP(hipGetLastError());
       pmeta = ctx->pmeta.as<ObjMeta>();
     }
