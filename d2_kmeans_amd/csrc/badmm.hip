@@ -876,7 +876,8 @@ void k_badmm_plan64(PlanArgs<T> a) {
   ((nc) <= 8 ? 6 : ((cm) == COST_HIST || (cm) == COST_SPARSE) ? 5 : 4)
 #endif
 #ifndef D2MI_PLANREG_RESID_WAVES
-#define D2MI_PLANREG_RESID_WAVES 4  // residual passes: 4 waves per SIMD (a small spill)
+// residual passes: 3 waves per SIMD for stored costs, 4 otherwise
+#define D2MI_PLANREG_RESID_WAVES(cm) ((cm) == COST_TABLE ? 3 : 4)
 #endif
 #ifndef D2MI_PLANREG_SETPRIO
 #define D2MI_PLANREG_SETPRIO 2  // wave priority while an object's loads are issued (0: none)
@@ -1017,7 +1018,7 @@ __attribute__((amdgpu_waves_per_eu(
     PF ? 4  // the staging arrays (36 KB) allow 4 workgroups per CU
     : MODE == 0 || (MODE == 1 && D2MI_PLANREG_MID_CAP && !std::is_same<T, double>::value)
         ? D2MI_PLANREG_WAVES_PER_EU(NC, CM)
-        : (MODE == 2 && !std::is_same<T, double>::value ? D2MI_PLANREG_RESID_WAVES : 1))))
+        : (MODE == 2 && !std::is_same<T, double>::value ? D2MI_PLANREG_RESID_WAVES(CM) : 1))))
 void k_badmm_plan_reg(PlanArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr bool F64 = std::is_same<T, double>::value;
