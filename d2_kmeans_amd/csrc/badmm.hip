@@ -876,8 +876,10 @@ void k_badmm_plan64(PlanArgs<T> a) {
   ((nc) <= 8 ? 6 : ((cm) == COST_HIST || (cm) == COST_SPARSE) ? 5 : 4)
 #endif
 #ifndef D2MI_PLANREG_RESID_WAVES
-// residual passes: 3 waves per SIMD for stored costs, 4 otherwise
-#define D2MI_PLANREG_RESID_WAVES(cm) ((cm) == COST_TABLE ? 3 : 4)
+// residual passes: 3 waves per SIMD for stored costs in workgroups of up to
+// 256 threads (no spill), 4 otherwise (a 512-thread workgroup puts 2 waves on
+// each SIMD, so 3 would hold one workgroup per CU instead of 2)
+#define D2MI_PLANREG_RESID_WAVES(cm, ntb) ((cm) == COST_TABLE && (ntb) <= 256 ? 3 : 4)
 #endif
 #ifndef D2MI_PLANREG_SETPRIO
 #define D2MI_PLANREG_SETPRIO 2  // wave priority while an object's loads are issued (0: none)
@@ -1018,7 +1020,7 @@ __attribute__((amdgpu_waves_per_eu(
     PF ? 4  // the staging arrays (36 KB) allow 4 workgroups per CU
     : MODE == 0 || (MODE == 1 && D2MI_PLANREG_MID_CAP && !std::is_same<T, double>::value)
         ? D2MI_PLANREG_WAVES_PER_EU(NC, CM)
-        : (MODE == 2 && !std::is_same<T, double>::value ? D2MI_PLANREG_RESID_WAVES(CM) : 1))))
+        : (MODE == 2 && !std::is_same<T, double>::value ? D2MI_PLANREG_RESID_WAVES(CM, NTB) : 1))))
 void k_badmm_plan_reg(PlanArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr bool F64 = std::is_same<T, double>::value;
